@@ -1,0 +1,204 @@
+/*
+ * thv.h -- C ABI of libthv_hip.so, the MI355X (gfx950) out-of-memory graph
+ * engine.  Plain C: opaque handles, plain pointers and sizes, no C++ or torch
+ * types.  Every entry point returns 0 (THV_OK) or a negative THV_E* code and
+ * never lets a C++ exception escape; thv_last_error() gives the message of the
+ * calling thread's last failure.
+ *
+ * Ownership: the library owns all device and pinned host memory; callers own
+ * every output buffer they pass in.  A thv_graph is not safe for concurrent
+ * algorithm calls (use one handle per thread); thv_serve_* is internally
+ * synchronised.
+ *
+ * Value conventions follow the reference (qqueke/Thievory):
+ *   BFS  out[v] = 1 + hops(src, v), out[src] = 1, unreached = n + 1
+ *   CC   out[v] = least vertex id that reaches v along out-edges
+ *   SSSP out[v] = min(n + 1, 1 + dist_w(src, v)), unreached = n + 1
+ *   PR   out[v] = unnormalised PageRank (1-a) + a * sum r[u]/outdeg(u)
+ * Citations are path:line in the reference.
+ */
+#ifndef THV_H
+#define THV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THV_OK 0
+#define THV_EINVAL (-1)  /* bad argument (null handle, src >= n, k > 64 ...) */
+#define THV_EIO (-2)     /* file cannot be opened or read */
+#define THV_ENOMEM (-3)  /* device or pinned host allocation failed */
+#define THV_EHIP (-4)    /* HIP runtime error */
+#define THV_EFORMAT (-5) /* file size does not match its header */
+#define THV_EBUSY (-6)   /* serving: no slot / scratch memory free */
+#define THV_ENOTSUP (-7) /* operation not supported for this graph */
+#define THV_ESTATE (-8)  /* handle in the wrong state (stopped server ...) */
+
+/* input file formats (byte layouts of converter/converter.cpp) */
+#define THV_FMT_CSR 0  /* u64 n,m; u64 off[n]; u32 dst[m]; [u32 w[m]]  converter.cpp:750-774 */
+#define THV_FMT_CSC 1  /* u64 n,m; u64 inoff[n]; u32 src[m]; u32 outdeg[n]  converter.cpp:572-584 */
+#define THV_FMT_WCSR 2 /* u64 n,m; u64 off[n]; {u32 dst,u32 w}[m]  converter.cpp:785-802 */
+
+#define THV_ALGO_BFS 0
+#define THV_ALGO_CC 1
+#define THV_ALGO_SSSP 2
+#define THV_ALGO_PR 3
+
+#define THV_PR_PUSH 0
+#define THV_PR_PULL 1
+
+typedef struct thv_graph thv_graph;
+typedef struct thv_server thv_server;
+
+/* Counters of one algorithm call.  Byte figures are algorithmic: what the
+ * algorithm must touch, not what the hardware moved (moved_bytes is the
+ * exception: bytes the engine actually copied over the host link by DMA). */
+typedef struct {
+  uint64_t edges_traversed; /* sum over iterations of active out-edges   */
+  uint64_t hbm_bytes;       /* device-resident edge records + state bytes */
+  uint64_t host_bytes;      /* host-resident edge records read            */
+  uint64_t moved_bytes;     /* bytes DMA'd host -> device                  */
+  uint64_t reached;         /* vertices with a finite value               */
+  uint32_t iterations;      /* levels / rounds until the frontier emptied */
+  uint32_t host_partitions_touched; /* host partitions read, summed      */
+  uint32_t host_partitions_skipped; /* host partitions skipped, summed    */
+  uint32_t reserved;
+  double ms;                /* first launch -> empty frontier (wall)      */
+  double expand_ms;         /* time inside edge-expansion kernels (events)*/
+} thv_stats;
+
+/* ---- graph lifetime --------------------------------------------------- */
+
+/* Opens a .csr/.csc/.wcsr file (graph.cuh:641-777).  edge_bytes 4 keeps
+ * destination-only records; 8 keeps {dst, w} records (weights from a .wcsr
+ * file or the trailing weight array of a .csr file; weight 20 if absent,
+ * the converter's default, converter.cpp:695).  Replaces
+ * CSR<uint32>::ReadInputFile + InitData. */
+int thv_graph_open(const char *path, int fmt, int edge_bytes, thv_graph **out);
+
+/* Builds a graph from host CSR arrays: off has n+1 entries, dst m, w m or
+ * NULL.  edge_bytes as above.  outdeg (n, nullable) marks the arrays as a
+ * CSC whose sources have these out-degrees (PageRank pull input). */
+int thv_graph_from_arrays(uint64_t n, uint64_t m, const uint64_t *off,
+                          const uint32_t *dst, const uint32_t *w,
+                          const uint32_t *outdeg, int edge_bytes,
+                          thv_graph **out);
+
+/* Generates a seeded R-MAT graph on the GPU: 2^scale vertices, edge_factor
+ * * 2^scale directed edges before clean-up (symmetric graphs draw half as
+ * many tuples and store each both ways), self-loops and duplicates removed,
+ * adjacency sorted.  weights != 0 attaches symmetric weights in [1, 64]
+ * (edge_bytes 8).  transpose != 0 builds the CSC of the directed graph
+ * instead (with out-degrees).  Bit-identical to the CPU generator. */
+int thv_graph_rmat(int scale, int edge_factor, double a, double b, double c,
+                   uint64_t seed, int symmetric, int weights, int transpose,
+                   thv_graph **out);
+
+int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
+                   int *edge_bytes, int *is_csc);
+
+/* Copies the CSR arrays to host buffers (any may be NULL). */
+int thv_graph_export(const thv_graph *g, uint64_t *off, uint32_t *dst,
+                     uint32_t *w, uint32_t *outdeg);
+
+void thv_graph_close(thv_graph *g);
+
+/* ---- split edge store ------------------------------------------------- */
+
+/* Splits the edge array into partitions of at most partition_edges edges
+ * (contiguous vertex ranges; one oversize vertex forms its own partition,
+ * graph.cuh:270-304) and keeps the highest-average-degree partitions in HBM
+ * up to device_edge_budget_bytes; the rest stays in mapped pinned host
+ * memory read zero-copy.  Replaces CSR::SetPartitionsConfig and
+ * CSR::SetNumStaticEdges (graph.cuh:306-341).  The default store (before any
+ * call) is fully device-resident with 2^16-edge partitions. */
+int thv_store_config(thv_graph *g, uint64_t device_edge_budget_bytes,
+                     uint32_t partition_edges);
+
+/* Store facts: partitions, partitions in HBM, edges in HBM. */
+int thv_store_info(const thv_graph *g, uint32_t *partitions,
+                   uint32_t *device_partitions, uint64_t *device_edges);
+
+/* ---- single-source / whole-graph algorithms --------------------------- */
+
+/* BFS levels (BFS32, bfs/bfs.cu:8-542).  out: n x u32 host buffer. */
+int thv_bfs(thv_graph *g, uint32_t src, uint32_t *out, thv_stats *st);
+
+/* Min-label propagation (CC32, cc/cc.cu:71-614). */
+int thv_cc(thv_graph *g, uint32_t *out, thv_stats *st);
+
+/* Frontier Bellman-Ford with atomic min (SSSP32, sssp/sssp.cu:10-570), with
+ * the graph's weights (edge_bytes 8). */
+int thv_sssp(thv_graph *g, uint32_t src, uint32_t *out, thv_stats *st);
+
+/* PageRank (PR32_PUSH pr/pr.cu:1030-1549 on a CSR graph, PR32 pull
+ * pr/pr.cu:11-519 on a CSC graph), fp32 ranks, stop when no vertex moves by
+ * more than tol or after max_iter iterations. */
+int thv_pr(thv_graph *g, int mode, float damping, float tol, int max_iter,
+           float *out, thv_stats *st);
+
+/* ---- batched multi-source queries (no reference counterpart) ---------- */
+
+/* Runs k <= 64 BFS (algo THV_ALGO_BFS) or SSSP (THV_ALGO_SSSP) queries in
+ * one persistent launch sequence: per-vertex 64-bit source masks for BFS,
+ * lane-packed distances for SSSP.  out_hash[k] receives the digest
+ *   sum over v with out[v] != n+1 of mix64((v << 32) | out[v])  (mod 2^64)
+ * where mix64 is the SplitMix64 output function, out_reached[k] the reached
+ * counts, out_depth[k] the largest finite value.  out_full (nullable) gets
+ * the k full value arrays, query-major (k x n). */
+int thv_batch(thv_graph *g, int algo, const uint32_t *srcs, uint32_t k,
+              uint64_t *out_hash, uint64_t *out_reached, uint32_t *out_depth,
+              uint32_t *out_full, thv_stats *st);
+
+/* Chooses the expansion launch shape (blocks per CU, edges per thread) for
+ * this graph by timing candidates; the choice is cached on the handle and
+ * written to cache_path (nullable).  chosen receives 2 ints. */
+int thv_autotune(thv_graph *g, const char *cache_path, int *chosen);
+
+/* ---- serving (continuous batching) ------------------------------------ */
+
+typedef struct {
+  uint32_t max_lanes;       /* concurrent queries per algorithm, <= 64     */
+  uint64_t scratch_bytes;   /* device memory admission budget (0 = auto)   */
+  uint32_t keep_full;       /* 1: keep full value arrays for thv_serve_poll */
+} thv_serve_cfg;
+
+typedef struct {
+  uint64_t ticket;
+  int32_t algo;
+  int32_t status;           /* 0 done, 1 pending, negative error          */
+  uint32_t src;
+  uint32_t depth;
+  uint64_t reached;
+  uint64_t hash;
+  double submit_ms, start_ms, done_ms; /* on the server's clock           */
+} thv_result;
+
+/* Starts a server thread that keeps the graph resident and admits queries
+ * into free lanes between levels of the running batch. */
+int thv_serve_start(thv_graph *g, const thv_serve_cfg *cfg, thv_server **out);
+/* Thread-safe; returns THV_EBUSY when the admission queue is full. */
+int thv_serve_submit(thv_server *s, int algo, uint32_t src, uint64_t *ticket);
+/* Thread-safe; status 1 while pending.  full (nullable, n entries) receives
+ * the value array when the server keeps them. */
+int thv_serve_poll(thv_server *s, uint64_t ticket, thv_result *res,
+                   uint32_t *full);
+/* Blocks until every submitted query has finished. */
+int thv_serve_drain(thv_server *s);
+void thv_serve_stop(thv_server *s);
+
+/* ---- misc ---------------------------------------------------------------- */
+
+const char *thv_last_error(void);
+/* Library version string and the device it runs on. */
+const char *thv_version(void);
+int thv_device_info(char *name, size_t name_len, int *cus,
+                    uint64_t *hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THV_H */

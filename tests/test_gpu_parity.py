@@ -1,0 +1,186 @@
+"""GPU parity of the HIP engine against the oracle (bit-exact for BFS, CC and
+SSSP; PageRank within 1e-5 normalised L1 of the FP64 fixpoint).  Every call
+goes through the C ABI of libthv_hip.so."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import thievory_amd as T
+
+pytestmark = pytest.mark.gpu
+
+PR_L1_TOL = 1e-5  # normalised L1 bound vs the FP64 oracle
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+# ---------------------------------------------------------------- fixtures ---
+
+def test_tiny4_hand_derived(golden):
+    _say("case: tiny4 fixture")
+    g = T.Graph.open(os.path.join(golden, "tiny4.csr"), "csr", 4)
+    assert (g.n, g.m) == (4, 6)
+    assert g.bfs(1)[0].tolist() == [3, 1, 2, 5]
+    assert g.cc()[0].tolist() == [0, 0, 0, 3]
+    g8 = T.Graph.open(os.path.join(golden, "tiny4.csr"), "csr", 8)
+    assert g8.sssp(1)[0].tolist() == [5, 1, 5, 5]
+    gw = T.Graph.open(os.path.join(golden, "tiny4.wcsr"), "wcsr", 8)
+    assert gw.sssp(1)[0].tolist() == [5, 1, 5, 5]
+    assert gw.bfs(1)[0].tolist() == [3, 1, 2, 5]
+
+
+@pytest.mark.parametrize("budget_frac", [1.0, 0.5, 0.0])
+def test_rand64_all_sources(golden, budget_frac):
+    _say("case: rand64 all sources, budget", budget_frac)
+    og = O.read_graph(os.path.join(golden, "rand64.csr"), "csr")
+    g = T.Graph.open(os.path.join(golden, "rand64.csr"), "csr", 8)
+    g.store_config(int(budget_frac * g.m * 8), 16)
+    assert g.store_info()["partitions"] > 1
+    for s in range(og.n):
+        assert np.array_equal(g.bfs(s)[0], O.bfs(og, s)), s
+        assert np.array_equal(g.sssp(s)[0], O.sssp(og, s)), s
+    assert np.array_equal(g.cc()[0], O.cc(og))
+
+
+def test_rand64_pr(golden):
+    _say("case: rand64 pagerank push/pull")
+    og = O.read_graph(os.path.join(golden, "rand64.csr"), "csr")
+    ref, _ = O.pr_exact(og)
+    g = T.Graph.open(os.path.join(golden, "rand64.csr"), "csr", 4)
+    push, _ = g.pr("push", tol=1e-7, max_iter=100)
+    assert O.normalized_l1(push, ref) < PR_L1_TOL
+    gc = T.Graph.open(os.path.join(golden, "rand64.csc"), "csc", 4)
+    pull, _ = gc.pr("pull", tol=1e-7, max_iter=100)
+    assert O.normalized_l1(pull, ref) < PR_L1_TOL
+
+
+# ------------------------------------------------------------- generator ---
+
+@pytest.mark.parametrize("scale,ef,sym,w", [(10, 8, True, True), (14, 16, True, True),
+                                           (12, 16, False, False)])
+def test_rmat_generator_matches_cpu(scale, ef, sym, w):
+    _say("case: rmat generator", scale, ef, sym, w)
+    g = T.Graph.rmat(scale, ef, seed=5, symmetric=sym, weights=w)
+    off, dst, ww, _ = g.export()
+    ref = O.rmat(scale, ef, seed=5, symmetric=sym, weights=w)
+    assert np.array_equal(off, ref.off)
+    assert np.array_equal(dst, ref.adj)
+    if w:
+        assert np.array_equal(ww, ref.w)
+
+
+def test_rmat_transpose_matches_cpu():
+    _say("case: rmat transpose")
+    g = T.Graph.rmat(12, 16, seed=3, symmetric=False, transpose=True)
+    off, src, _, od = g.export()
+    ref = O.transpose(O.rmat(12, 16, seed=3, symmetric=False))
+    assert np.array_equal(off, ref.off)
+    assert np.array_equal(src, ref.adj)
+    assert np.array_equal(od, ref.outdeg)
+
+
+# ---------------------------------------------------------- split store ---
+
+@pytest.fixture(scope="module")
+def rmat16():
+    g = T.Graph.rmat(16, 16, seed=1, weights=True)
+    og = O.rmat(16, 16, seed=1, weights=True)
+    return g, og
+
+
+@pytest.mark.parametrize("frac,part", [(1.0, 1 << 16), (0.25, 1 << 16), (0.25, 1024),
+                                       (0.0, 4096)])
+def test_bfs_cc_sssp_split_store(rmat16, frac, part):
+    g, og = rmat16
+    _say("case: rmat16 split store", frac, part)
+    g.store_config(int(frac * g.m * g.edge_bytes), part)
+    info = g.store_info()
+    if frac == 0.0:
+        assert info["device_edges"] == 0
+    for s in (1, 7, 12345):
+        lv, st = g.bfs(s)
+        ref = O.bfs(og, s)
+        assert np.array_equal(lv, ref), s
+        reached = ref != og.n + 1
+        assert st.edges_traversed == int(og.degrees()[reached].sum())
+        if frac == 1.0:
+            assert st.host_bytes == 0
+        d, _ = g.sssp(s)
+        assert np.array_equal(d, O.sssp(og, s)), s
+    assert np.array_equal(g.cc()[0], O.cc(og))
+
+
+def test_pr_push_pull_rmat():
+    _say("case: rmat14 directed pagerank")
+    og = O.rmat(14, 16, seed=2, symmetric=False)
+    ref, _ = O.pr_exact(og)
+    g = T.Graph.rmat(14, 16, seed=2, symmetric=False)
+    g.store_config(int(0.25 * g.m * 4), 1 << 12)
+    push, st = g.pr("push")
+    assert O.normalized_l1(push, ref) < PR_L1_TOL, O.normalized_l1(push, ref)
+    gc = T.Graph.rmat(14, 16, seed=2, symmetric=False, transpose=True)
+    gc.store_config(int(0.25 * gc.m * 4), 1 << 12)
+    pull, st2 = gc.pr("pull")
+    assert O.normalized_l1(pull, ref) < PR_L1_TOL, O.normalized_l1(pull, ref)
+
+
+# ----------------------------------------------------------- edge cases ---
+
+def test_source_out_of_range(golden):
+    _say("case: bad source")
+    g = T.Graph.open(os.path.join(golden, "tiny4.csr"), "csr", 4)
+    with pytest.raises(T.ThvError) as e:
+        g.bfs(4)
+    assert e.value.code == -1
+
+
+def test_sssp_needs_weights(golden):
+    g = T.Graph.open(os.path.join(golden, "tiny4.csr"), "csr", 4)
+    with pytest.raises(T.ThvError) as e:
+        g.sssp(1)
+    assert e.value.code == -7
+
+
+def test_single_vertex_and_isolated():
+    _say("case: single vertex / isolated")
+    g = T.Graph.from_arrays(np.array([0, 0], np.uint64), np.zeros(0, np.uint32))
+    assert g.bfs(0)[0].tolist() == [1]
+    assert g.cc()[0].tolist() == [0]
+    off = np.array([0, 1, 1, 1, 2], np.uint64)
+    dst = np.array([3, 0], np.uint32)
+    g = T.Graph.from_arrays(off, dst)
+    assert g.bfs(0)[0].tolist() == [1, 5, 5, 2]
+    assert g.cc()[0].tolist() == [0, 1, 2, 0]
+
+
+def test_hub_spanning_many_chunks():
+    """A star whose hub list spans many expansion chunks, plus a long path
+    (many levels)."""
+    _say("case: star + path")
+    n = 22000
+    # hub 0 -> 1..19999, path 19999 -> 20000 -> ... -> 21999
+    adj = [list(range(1, 20000))] + [[] for _ in range(n - 1)]
+    adj[19999].append(20000)
+    for v in range(20000, n - 1):
+        adj[v].append(v + 1)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(a) for a in adj])
+    dst = np.array([x for a in adj for x in a], np.uint32)
+    og = O.CSR(off, dst)
+    g = T.Graph.from_arrays(off, dst)
+    for frac in (1.0, 0.0):
+        g.store_config(int(frac * g.m * 4), 512)
+        assert np.array_equal(g.bfs(0)[0], O.bfs(og, 0))
+        assert np.array_equal(g.cc()[0], O.cc(og))
+
+
+def test_directed_cc_is_min_ancestor():
+    _say("case: directed cc")
+    og = O.rmat(12, 8, seed=9, symmetric=False)
+    g = T.Graph.from_arrays(og.off, og.adj)
+    g.store_config(int(0.3 * g.m * 4), 256)
+    assert np.array_equal(g.cc()[0], O.cc_minreach(og))

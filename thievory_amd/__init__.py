@@ -1,0 +1,290 @@
+"""thievory_amd -- MI355X-native out-of-memory graph engine (host mirror).
+
+A ctypes binding of libthv_hip.so (include/thv.h).  The engine itself is
+native: HIP kernels for gfx950 behind a C ABI.  This module only moves
+arguments and numpy buffers across that boundary; it has no compute path of
+its own and raises if the native library is missing.
+
+    g = Graph.rmat(20, 16)                 # or Graph.open("g.csr", "csr")
+    g.store_config(budget_bytes=..., partition_edges=1 << 16)
+    levels, stats = g.bfs(1)
+
+Value conventions are the reference's (qqueke/Thievory): BFS level = 1 +
+hops with n+1 for unreached (src/graph.cuh:240-246), CC = least id reaching
+the vertex (src/cc/cc_kernels.cu:50-66), SSSP = min(n+1, 1 + distance),
+PageRank unnormalised (src/pr/pr_kernels.cu:449-476, 846-860).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libthv_hip.so")
+CLI_PATH = os.path.join(HERE, "_lib", "thv_main")
+
+FMT = {"csr": 0, "csc": 1, "wcsr": 2}
+ALGO = {"bfs": 0, "cc": 1, "sssp": 2, "pr": 3}
+PR_MODE = {"push": 0, "pull": 1}
+ERRORS = {0: "OK", -1: "EINVAL", -2: "EIO", -3: "ENOMEM", -4: "EHIP", -5: "EFORMAT",
+          -6: "EBUSY", -7: "ENOTSUP", -8: "ESTATE"}
+
+RMAT_ABC = (0.57, 0.19, 0.19)
+
+
+class ThvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [("edges_traversed", C.c_uint64), ("hbm_bytes", C.c_uint64),
+                ("host_bytes", C.c_uint64), ("moved_bytes", C.c_uint64),
+                ("reached", C.c_uint64), ("iterations", C.c_uint32),
+                ("host_partitions_touched", C.c_uint32),
+                ("host_partitions_skipped", C.c_uint32), ("reserved", C.c_uint32),
+                ("ms", C.c_double), ("expand_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ServeCfg(C.Structure):
+    _fields_ = [("max_lanes", C.c_uint32), ("scratch_bytes", C.c_uint64),
+                ("keep_full", C.c_uint32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("algo", C.c_int32), ("status", C.c_int32),
+                ("src", C.c_uint32), ("depth", C.c_uint32), ("reached", C.c_uint64),
+                ("hash", C.c_uint64), ("submit_ms", C.c_double), ("start_ms", C.c_double),
+                ("done_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_vp = C.c_void_p
+
+
+def lib():
+    """Loads the native library; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m thievory_amd.build` "
+                          "(the engine has no non-native fallback)")
+    L = C.CDLL(LIB_PATH)
+    G = C.POINTER(_vp)
+    sig = {
+        "thv_graph_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, G]),
+        "thv_graph_from_arrays": (C.c_int, [C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp,
+                                            C.c_int, G]),
+        "thv_graph_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_int, C.c_int, C.c_int, G]),
+        "thv_graph_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "thv_graph_export": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "thv_graph_close": (None, [_vp]),
+        "thv_store_config": (C.c_int, [_vp, C.c_uint64, C.c_uint32]),
+        "thv_store_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint64)]),
+        "thv_bfs": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(Stats)]),
+        "thv_cc": (C.c_int, [_vp, _vp, C.POINTER(Stats)]),
+        "thv_sssp": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(Stats)]),
+        "thv_pr": (C.c_int, [_vp, C.c_int, C.c_float, C.c_float, C.c_int, _vp,
+                             C.POINTER(Stats)]),
+        "thv_batch": (C.c_int, [_vp, C.c_int, _vp, C.c_uint32, _vp, _vp, _vp, _vp,
+                                C.POINTER(Stats)]),
+        "thv_autotune": (C.c_int, [_vp, C.c_char_p, _vp]),
+        "thv_serve_start": (C.c_int, [_vp, C.POINTER(ServeCfg), G]),
+        "thv_serve_submit": (C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_uint64)]),
+        "thv_serve_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(Result), _vp]),
+        "thv_serve_drain": (C.c_int, [_vp]),
+        "thv_serve_stop": (None, [_vp]),
+        "thv_last_error": (C.c_char_p, []),
+        "thv_version": (C.c_char_p, []),
+        "thv_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().thv_last_error()
+        raise ThvError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def device_info():
+    name = C.create_string_buffer(64)
+    cus, hbm = C.c_int(), C.c_uint64()
+    check(lib().thv_device_info(name, 64, C.byref(cus), C.byref(hbm)))
+    return {"arch": name.value.decode(), "cus": cus.value, "hbm_bytes": hbm.value}
+
+
+class Graph:
+    """A graph resident in the engine: pinned host edge array, per-vertex
+    tables and the HBM share of the split edge store."""
+
+    def __init__(self, handle):
+        self._h = handle
+        n, m, eb, csc = C.c_uint64(), C.c_uint64(), C.c_int(), C.c_int()
+        check(lib().thv_graph_info(self._h, C.byref(n), C.byref(m), C.byref(eb), C.byref(csc)))
+        self.n, self.m, self.edge_bytes, self.is_csc = n.value, m.value, eb.value, bool(csc.value)
+
+    # ---- construction -------------------------------------------------------
+    @classmethod
+    def open(cls, path, fmt="csr", edge_bytes=4):
+        h = _vp()
+        check(lib().thv_graph_open(os.fsencode(path), FMT[fmt], edge_bytes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, off, dst, w=None, outdeg=None, edge_bytes=None):
+        off = np.ascontiguousarray(off, np.uint64)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, np.uint32)
+        outdeg = None if outdeg is None else np.ascontiguousarray(outdeg, np.uint32)
+        if edge_bytes is None:
+            edge_bytes = 8 if w is not None else 4
+        h = _vp()
+        check(lib().thv_graph_from_arrays(len(off) - 1, len(dst), _p(off), _p(dst), _p(w),
+                                          _p(outdeg), edge_bytes, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, seed=1, symmetric=True, weights=False, transpose=False,
+             abc=RMAT_ABC):
+        h = _vp()
+        check(lib().thv_graph_rmat(scale, edge_factor, abc[0], abc[1], abc[2], seed,
+                                   int(symmetric), int(weights), int(transpose), C.byref(h)))
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            lib().thv_graph_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- store ----------------------------------------------------------------
+    def store_config(self, budget_bytes=None, partition_edges=1 << 16):
+        b = (1 << 64) - 1 if budget_bytes is None else int(budget_bytes)
+        check(lib().thv_store_config(self._h, b, partition_edges))
+
+    def store_info(self):
+        p, pd, de = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().thv_store_info(self._h, C.byref(p), C.byref(pd), C.byref(de)))
+        return {"partitions": p.value, "device_partitions": pd.value,
+                "device_edges": de.value}
+
+    def export(self):
+        off = np.zeros(self.n + 1, np.uint64)
+        dst = np.zeros(self.m, np.uint32)
+        w = np.zeros(self.m, np.uint32) if self.edge_bytes == 8 else None
+        od = np.zeros(self.n, np.uint32) if self.is_csc else None
+        check(lib().thv_graph_export(self._h, _p(off), _p(dst), _p(w), _p(od)))
+        return off, dst, w, od
+
+    # ---- algorithms -------------------------------------------------------------
+    def bfs(self, src, out=None):
+        out = np.zeros(self.n, np.uint32) if out is None else out
+        st = Stats()
+        check(lib().thv_bfs(self._h, src, _p(out), C.byref(st)))
+        return out, st
+
+    def cc(self, out=None):
+        out = np.zeros(self.n, np.uint32) if out is None else out
+        st = Stats()
+        check(lib().thv_cc(self._h, _p(out), C.byref(st)))
+        return out, st
+
+    def sssp(self, src, out=None):
+        out = np.zeros(self.n, np.uint32) if out is None else out
+        st = Stats()
+        check(lib().thv_sssp(self._h, src, _p(out), C.byref(st)))
+        return out, st
+
+    def pr(self, mode="push", damping=0.85, tol=1e-7, max_iter=100, out=None):
+        out = np.zeros(self.n, np.float32) if out is None else out
+        st = Stats()
+        check(lib().thv_pr(self._h, PR_MODE[mode], damping, tol, max_iter, _p(out),
+                           C.byref(st)))
+        return out, st
+
+    def batch(self, algo, srcs, full=False):
+        srcs = np.ascontiguousarray(srcs, np.uint32)
+        k = len(srcs)
+        h = np.zeros(k, np.uint64)
+        r = np.zeros(k, np.uint64)
+        d = np.zeros(k, np.uint32)
+        f = np.zeros((k, self.n), np.uint32) if full else None
+        st = Stats()
+        check(lib().thv_batch(self._h, ALGO[algo], _p(srcs), k, _p(h), _p(r), _p(d), _p(f),
+                              C.byref(st)))
+        return {"hash": h, "reached": r, "depth": d, "full": f, "stats": st}
+
+    def autotune(self, cache_path=None):
+        ch = np.zeros(2, np.int32)
+        check(lib().thv_autotune(self._h, None if cache_path is None else os.fsencode(cache_path),
+                                 _p(ch)))
+        return {"blocks_per_cu": int(ch[0]), "edges_per_thread": int(ch[1])}
+
+
+class Server:
+    """Continuous-batching query server over a resident graph."""
+
+    def __init__(self, graph, max_lanes=64, scratch_bytes=0, keep_full=False):
+        self.graph = graph
+        cfg = ServeCfg(max_lanes, scratch_bytes, 1 if keep_full else 0)
+        h = _vp()
+        check(lib().thv_serve_start(graph._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def submit(self, algo, src):
+        t = C.c_uint64()
+        check(lib().thv_serve_submit(self._h, ALGO[algo], src, C.byref(t)))
+        return t.value
+
+    def poll(self, ticket, full=None):
+        r = Result()
+        check(lib().thv_serve_poll(self._h, ticket, C.byref(r), _p(full)))
+        return r.as_dict()
+
+    def drain(self):
+        check(lib().thv_serve_drain(self._h))
+
+    def stop(self):
+        if self._h:
+            lib().thv_serve_stop(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -1,0 +1,199 @@
+// abi.cpp -- the extern "C" boundary of libthv_hip.so (include/thv.h).
+// Every entry point converts exceptions to THV_E* codes and records the
+// message for thv_last_error() in thread-local storage.
+#include <new>
+
+#include "internal.hpp"
+
+namespace thv {
+void set_last_error(const std::string &m);
+const char *last_error();
+Graph *graph_open(const char *path, int fmt, int edge_bytes);
+Graph *graph_from_arrays(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *dst,
+                         const uint32_t *w, const uint32_t *outdeg, int edge_bytes);
+void graph_export(const Graph &g, uint64_t *off, uint32_t *dst, uint32_t *w, uint32_t *outdeg);
+}  // namespace thv
+
+using namespace thv;
+
+template <class F>
+static int guarded(F f) {
+  try {
+    f();
+    return THV_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return THV_ENOMEM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return THV_EHIP;
+  } catch (...) {
+    set_last_error("unknown error");
+    return THV_EHIP;
+  }
+}
+
+#define NEED(x)                                               \
+  do {                                                        \
+    if (!(x)) fail(THV_EINVAL, "invalid argument: " #x);      \
+  } while (0)
+
+extern "C" {
+
+int thv_graph_open(const char *path, int fmt, int edge_bytes, thv_graph **out) {
+  return guarded([&] {
+    NEED(out);
+    *out = reinterpret_cast<thv_graph *>(graph_open(path, fmt, edge_bytes));
+  });
+}
+
+int thv_graph_from_arrays(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *dst,
+                          const uint32_t *w, const uint32_t *outdeg, int edge_bytes,
+                          thv_graph **out) {
+  return guarded([&] {
+    NEED(out);
+    *out = reinterpret_cast<thv_graph *>(
+        graph_from_arrays(n, m, off, dst, w, outdeg, edge_bytes));
+  });
+}
+
+int thv_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                   int symmetric, int weights, int transpose, thv_graph **out) {
+  return guarded([&] {
+    NEED(out);
+    *out = reinterpret_cast<thv_graph *>(rmat_generate(scale, edge_factor, a, b, c, seed,
+                                                       symmetric != 0, weights != 0,
+                                                       transpose != 0));
+  });
+}
+
+int thv_graph_info(const thv_graph *gh, uint64_t *n, uint64_t *m, int *edge_bytes, int *is_csc) {
+  return guarded([&] {
+    NEED(gh);
+    const Graph *g = reinterpret_cast<const Graph *>(gh);
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (edge_bytes) *edge_bytes = g->rec;
+    if (is_csc) *is_csc = g->csc ? 1 : 0;
+  });
+}
+
+int thv_graph_export(const thv_graph *gh, uint64_t *off, uint32_t *dst, uint32_t *w,
+                     uint32_t *outdeg) {
+  return guarded([&] {
+    NEED(gh);
+    graph_export(*reinterpret_cast<const Graph *>(gh), off, dst, w, outdeg);
+  });
+}
+
+void thv_graph_close(thv_graph *gh) {
+  if (!gh) return;
+  try {
+    delete reinterpret_cast<Graph *>(gh);
+  } catch (...) {
+  }
+}
+
+int thv_store_config(thv_graph *gh, uint64_t budget, uint32_t partition_edges) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    store_configure(*g, budget, partition_edges);
+  });
+}
+
+int thv_store_info(const thv_graph *gh, uint32_t *partitions, uint32_t *device_partitions,
+                   uint64_t *device_edges) {
+  return guarded([&] {
+    NEED(gh);
+    const Graph *g = reinterpret_cast<const Graph *>(gh);
+    if (partitions) *partitions = g->store.P;
+    if (device_partitions) *device_partitions = g->store.Pdev;
+    if (device_edges) *device_edges = g->store.dev_edges;
+  });
+}
+
+int thv_bfs(thv_graph *gh, uint32_t src, uint32_t *out, thv_stats *st) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_bfs(*g, src, out, st);
+  });
+}
+
+int thv_cc(thv_graph *gh, uint32_t *out, thv_stats *st) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_cc(*g, out, st);
+  });
+}
+
+int thv_sssp(thv_graph *gh, uint32_t src, uint32_t *out, thv_stats *st) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_sssp(*g, src, out, st);
+  });
+}
+
+int thv_pr(thv_graph *gh, int mode, float damping, float tol, int max_iter, float *out,
+           thv_stats *st) {
+  return guarded([&] {
+    NEED(gh);
+    NEED(damping > 0.0f && damping < 1.0f);
+    NEED(tol >= 0.0f);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_pr(*g, mode, damping, tol, max_iter, out, st);
+  });
+}
+
+int thv_batch(thv_graph *gh, int algo, const uint32_t *srcs, uint32_t k, uint64_t *out_hash,
+              uint64_t *out_reached, uint32_t *out_depth, uint32_t *out_full, thv_stats *st) {
+  return guarded([&] {
+    NEED(gh);
+    NEED(srcs || k == 0);
+    NEED(k <= 64);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_batch(*g, algo, srcs, k, out_hash, out_reached, out_depth, out_full, st);
+  });
+}
+
+int thv_autotune(thv_graph *gh, const char *cache_path, int *chosen) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    autotune(*g, cache_path, chosen);
+  });
+}
+
+const char *thv_last_error(void) { return last_error(); }
+
+const char *thv_version(void) { return "thievory-amd 0.1 (gfx950)"; }
+
+int thv_device_info(char *name, size_t name_len, int *cus, uint64_t *hbm_bytes) {
+  return guarded([&] {
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPC(hipGetDeviceProperties(&p, dev));
+    if (name && name_len) {
+      strncpy(name, p.gcnArchName, name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (cus) *cus = p.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,371 @@
+// algos.hip -- host iteration loops of BFS, CC, SSSP and PageRank.
+//
+// Reference drivers: BFS32 (src/bfs/bfs.cu:8-542), CC32 (src/cc/cc.cu:
+// 71-614), SSSP32 (src/sssp/sssp.cu:10-570), PR32 pull (src/pr/pr.cu:
+// 11-519) and PR32_PUSH (src/pr/pr.cu:1030-1549).  Each reference level does
+// about 8 O(n) passes and 6-10 blocking host syncs; here a level is
+// compaction (3 kernels) + expansion (1 kernel) + one 48-byte readback.
+// Initial values follow CSR::InitFrontierNValues (src/graph.cuh:229-268).
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+#include "scan.hpp"
+
+namespace thv {
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_iota_u32(uint32_t *p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
+// Bitmap with bits [0, n) set and every later bit of the `words` words clear.
+__global__ void k_bits_prefix(uint32_t *bm, uint64_t n, uint64_t words) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < words;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = i * 32;
+    uint32_t x = 0;
+    if (lo + 32 <= n) x = 0xFFFFFFFFu;
+    else if (lo < n) x = (1u << (n - lo)) - 1u;
+    bm[i] = x;
+  }
+}
+
+__global__ void k_set_bit(uint32_t *bm, uint32_t v) { bm[v >> 5] |= 1u << (v & 31); }
+
+static int fill_grid(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, 8192));
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(n)), dim3(256), 0, s, p, v, n);
+}
+
+// ---------------------------------------------------------------------------
+// PageRank update kernels.  One wave covers 64 consecutive vertices, so the
+// next-frontier words are written whole from a ballot (no atomics), and the
+// number of activated vertices is summed into aux[0].
+
+// Push (pr_kernels.cu:846-860): residual above tol is applied and re-pushed.
+__global__ void __launch_bounds__(256)
+k_pr_push_update(uint64_t n, float alpha, float tol, const uint32_t *__restrict__ deg,
+                 float *rank, float *delta, float *res, uint32_t *next,
+                 unsigned long long *aux) {
+  const uint64_t words64 = (n + 63) / 64;
+  for (uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull;
+       base < words64 * 64; base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = base + lane_id();
+    bool act = false;
+    if (v < n) {
+      const float r = res[v];
+      if (r > tol) {
+        const uint32_t d = deg[v];
+        rank[v] += r;
+        delta[v] = d ? r * alpha / (float)d : 0.0f;
+        res[v] = 0.0f;
+        act = true;
+      }
+    }
+    const uint64_t b = __ballot(act);
+    if (lane_id() == 0) {
+      next[base / 32] = (uint32_t)b;
+      next[base / 32 + 1] = (uint32_t)(b >> 32);
+      if (b) atomicAdd(aux, (unsigned long long)__popcll(b));
+    }
+  }
+}
+
+// Pull (pr_kernels.cu:449-476): active vertices take their new sum and stay
+// active while they moved by more than tol.
+__global__ void __launch_bounds__(256)
+k_pr_pull_update(uint64_t n, float alpha, float tol, const uint32_t *__restrict__ outdeg,
+                 const uint32_t *__restrict__ cur, float *rank, float *contrib, float *sum,
+                 uint32_t *next, unsigned long long *aux) {
+  const uint64_t words64 = (n + 63) / 64;
+  for (uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull;
+       base < words64 * 64; base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = base + lane_id();
+    bool act = false;
+    if (v < n && ((cur[v >> 5] >> (v & 31)) & 1u)) {
+      const float nr = (1.0f - alpha) + alpha * sum[v];
+      const float d = fabsf(nr - rank[v]);
+      rank[v] = nr;
+      const uint32_t od = outdeg[v];
+      contrib[v] = od ? nr / (float)od : 0.0f;
+      sum[v] = 0.0f;
+      act = d > tol;
+    }
+    const uint64_t b = __ballot(act);
+    if (lane_id() == 0) {
+      next[base / 32] = (uint32_t)b;
+      next[base / 32 + 1] = (uint32_t)(b >> 32);
+      if (b) atomicAdd(aux, (unsigned long long)__popcll(b));
+    }
+  }
+}
+
+__global__ void k_pr_push_init(uint64_t n, float alpha, const uint32_t *__restrict__ deg,
+                               float *rank, float *delta, float *res) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = deg[v];
+    rank[v] = 1.0f - alpha;
+    delta[v] = d ? (1.0f - alpha) * alpha / (float)d : 0.0f;
+    res[v] = 0.0f;
+  }
+}
+
+__global__ void k_pr_pull_init(uint64_t n, const uint32_t *__restrict__ outdeg, float *rank,
+                               float *contrib, float *sum) {
+  const float r0 = 1.0f / (float)n;
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t od = outdeg[v];
+    rank[v] = r0;
+    contrib[v] = od ? r0 / (float)od : 0.0f;
+    sum[v] = 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+static void stats_begin(thv_stats *st) {
+  if (st) memset(st, 0, sizeof *st);
+}
+
+static void stats_level(Graph &g, thv_stats *st, const LevelCtr &h) {
+  if (!st) return;
+  const uint64_t rec = (uint64_t)g.rec;
+  const uint64_t words = g.n / 32 + 2;
+  const uint64_t e = h.qw[0] + h.qw[1];
+  st->edges_traversed += e;
+  st->host_bytes += h.qw[0] * rec;
+  // device-resident records, one 4-byte state access per edge, 12 bytes
+  // (degree, first-record address half, own value) per active vertex, and
+  // the frontier bitmap read + cleared once per level.
+  st->hbm_bytes += h.qw[1] * rec + 4 * e + 12 * (h.qn[0] + h.qn[1]) + 8 * words;
+  const uint32_t host_parts = g.store.P - g.store.Pdev;
+  st->host_partitions_touched += h.touched[0];
+  st->host_partitions_skipped += host_parts - std::min(host_parts, h.touched[0]);
+  st->iterations++;
+}
+
+static ExpandArgs base_args(Graph &g) {
+  Workspace &w = g.ws;
+  ExpandArgs a;
+  memset(&a, 0, sizeof a);
+  for (int q = 0; q < 2; q++) {
+    a.qv[q] = w.qv[q].as<uint32_t>();
+    a.qe[q] = w.qe[q].as<unsigned long long>();
+    a.qc[q] = w.qc[q].as<uint32_t>();
+  }
+  a.eptr = g.store.d_eptr.as<uint64_t>();
+  a.val = w.val.as<uint32_t>();
+  a.vis = w.vis.as<uint32_t>();
+  a.C = w.C;
+  a.rec = g.rec;
+  return a;
+}
+
+// Level-synchronous loop shared by BFS, CC and SSSP.  front[0] holds the
+// first frontier on entry.
+static void value_loop(Graph &g, Algo algo, thv_stats *st) {
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t words = g.n / 32 + 2;
+  ExpandArgs a = base_args(g);
+  LevelCtr *d_ctr = w.ctr.as<LevelCtr>();
+  int cur = 0;
+  double expand_ms = 0;
+  const uint64_t t0 = now_ns();
+  for (uint32_t L = 0;; L++) {
+    LevelCtr *c = d_ctr + (L & 1);
+    launch_compact(g, w.front[cur].as<uint32_t>(), c, s);
+    HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
+    a.ctr = c;
+    a.next = w.front[cur ^ 1].as<uint32_t>();
+    a.level_value = L + 2;
+    HIPC(hipEventRecord(w.ev[0], s));
+    launch_expand(g, algo, a, s);
+    HIPC(hipEventRecord(w.ev[1], s));
+    HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const LevelCtr &h = w.h_ctr[L & 1];
+    if (h.qn[0] + h.qn[1] == 0) break;
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    expand_ms += ms;
+    stats_level(g, st, h);
+    cur ^= 1;
+  }
+  if (st) {
+    st->ms = (now_ns() - t0) * 1e-6;
+    st->expand_ms = expand_ms;
+  }
+}
+
+static void finish_values(Graph &g, uint32_t *out, thv_stats *st, uint32_t sentinel) {
+  if (out && g.n)
+    HIPC(hipMemcpy(out, g.ws.val.p, g.n * 4, hipMemcpyDeviceToHost));
+  if (st && out) {
+    uint64_t r = 0;
+    for (uint64_t v = 0; v < g.n; v++) r += out[v] != sentinel;
+    st->reached = r;
+  }
+}
+
+static void check_src(Graph &g, uint32_t src) {
+  if (src >= g.n)
+    fail(THV_EINVAL, "source vertex " + std::to_string(src) + " out of range (n = " +
+                         std::to_string(g.n) + ")");
+}
+
+void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
+  check_src(g, src);
+  workspace_ensure(g);
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t words = g.n / 32 + 2;
+  stats_begin(st);
+  launch_fill_u32(w.val.as<uint32_t>(), (uint32_t)(g.n + 1), g.n, s);
+  launch_fill_u32(w.val.as<uint32_t>() + src, 1u, 1, s);
+  HIPC(hipMemsetAsync(w.vis.p, 0, words * 4, s));
+  HIPC(hipMemsetAsync(w.front[0].p, 0, words * 4, s));
+  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.vis.as<uint32_t>(), src);
+  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
+  HIPC(hipStreamSynchronize(s));
+  value_loop(g, A_BFS, st);
+  finish_values(g, out, st, (uint32_t)(g.n + 1));
+}
+
+void run_cc(Graph &g, uint32_t *out, thv_stats *st) {
+  workspace_ensure(g);
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t words = g.n / 32 + 2;
+  stats_begin(st);
+  if (g.n) {
+    hipLaunchKernelGGL(k_iota_u32, dim3(fill_grid(g.n)), dim3(256), 0, s, w.val.as<uint32_t>(),
+                       g.n);
+  }
+  hipLaunchKernelGGL(k_bits_prefix, dim3(fill_grid(words)), dim3(256), 0, s,
+                     w.front[0].as<uint32_t>(), g.n, words);
+  HIPC(hipStreamSynchronize(s));
+  value_loop(g, A_CC, st);
+  finish_values(g, out, st, 0xFFFFFFFFu);
+}
+
+void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
+  check_src(g, src);
+  if (g.rec != 8) fail(THV_ENOTSUP, "SSSP needs a weighted graph (8-byte records)");
+  workspace_ensure(g);
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t words = g.n / 32 + 2;
+  stats_begin(st);
+  launch_fill_u32(w.val.as<uint32_t>(), (uint32_t)(g.n + 1), g.n, s);
+  launch_fill_u32(w.val.as<uint32_t>() + src, 1u, 1, s);
+  HIPC(hipMemsetAsync(w.front[0].p, 0, words * 4, s));
+  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
+  HIPC(hipStreamSynchronize(s));
+  value_loop(g, A_SSSP, st);
+  finish_values(g, out, st, (uint32_t)(g.n + 1));
+}
+
+void run_pr(Graph &g, int mode, float alpha, float tol, int max_iter, float *out,
+            thv_stats *st) {
+  if (mode != THV_PR_PUSH && mode != THV_PR_PULL) fail(THV_EINVAL, "bad PageRank mode");
+  if (mode == THV_PR_PULL && !g.csc)
+    fail(THV_ENOTSUP, "PageRank pull needs a CSC graph (in-edges + out-degrees)");
+  if (mode == THV_PR_PUSH && g.csc)
+    fail(THV_ENOTSUP, "PageRank push needs a CSR graph (out-edges)");
+  if (max_iter < 0) fail(THV_EINVAL, "max_iter must be >= 0");
+  workspace_ensure(g);
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t n = g.n;
+  const uint64_t words = n / 32 + 2;
+  if (w.f0.bytes < (n + 1) * 4) {
+    w.f0.alloc((n + 1) * 4);
+    w.f1.alloc((n + 1) * 4);
+    w.f2.alloc((n + 1) * 4);
+  }
+  stats_begin(st);
+  float *rank = w.f0.as<float>(), *f1 = w.f1.as<float>(), *f2 = w.f2.as<float>();
+  const int fg = fill_grid(n);
+  if (n) {
+    if (mode == THV_PR_PUSH)
+      hipLaunchKernelGGL(k_pr_push_init, dim3(fg), dim3(256), 0, s, n, alpha,
+                         g.d_deg.as<uint32_t>(), rank, f1, f2);
+    else
+      hipLaunchKernelGGL(k_pr_pull_init, dim3(fg), dim3(256), 0, s, n,
+                         g.d_outdeg.as<uint32_t>(), rank, f1, f2);
+  }
+  hipLaunchKernelGGL(k_bits_prefix, dim3(fill_grid(words)), dim3(256), 0, s,
+                     w.front[0].as<uint32_t>(), n, words);
+  HIPC(hipMemsetAsync(w.front[1].p, 0, words * 4, s));
+  HIPC(hipStreamSynchronize(s));
+
+  ExpandArgs a = base_args(g);
+  a.pr_src = f1;  // push: delta, pull: contribution
+  a.pr_dst = f2;  // push: residual, pull: sum
+  LevelCtr *d_ctr = w.ctr.as<LevelCtr>();
+  unsigned long long *aux = w.aux.as<unsigned long long>();
+  const Algo algo = mode == THV_PR_PUSH ? A_PR_PUSH : A_PR_PULL;
+  int cur = 0;
+  double expand_ms = 0;
+  const uint64_t t0 = now_ns();
+  const int ug = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 4096));
+  for (int it = 0; it < max_iter; it++) {
+    LevelCtr *c = d_ctr + (it & 1);
+    launch_compact(g, w.front[cur].as<uint32_t>(), c, s);
+    a.ctr = c;
+    HIPC(hipEventRecord(w.ev[0], s));
+    launch_expand(g, algo, a, s);
+    HIPC(hipEventRecord(w.ev[1], s));
+    HIPC(hipMemsetAsync(aux, 0, 8, s));
+    if (mode == THV_PR_PUSH)
+      hipLaunchKernelGGL(k_pr_push_update, dim3(ug), dim3(256), 0, s, n, alpha, tol,
+                         g.d_deg.as<uint32_t>(), rank, f1, f2,
+                         w.front[cur ^ 1].as<uint32_t>(), aux);
+    else
+      hipLaunchKernelGGL(k_pr_pull_update, dim3(ug), dim3(256), 0, s, n, alpha, tol,
+                         g.d_outdeg.as<uint32_t>(), w.front[cur].as<uint32_t>(), rank, f1,
+                         f2, w.front[cur ^ 1].as<uint32_t>(), aux);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(&w.h_ctr[it & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(w.h_aux, aux, 8, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    expand_ms += ms;
+    const LevelCtr &h = w.h_ctr[it & 1];
+    stats_level(g, st, h);
+    if (st) {
+      // push: 4-byte atomic read+write of the residual per edge in place of
+      // the state read; pull: 4-byte contribution read per edge.
+      const uint64_t e = h.qw[0] + h.qw[1];
+      st->hbm_bytes += (mode == THV_PR_PUSH ? 4 : 0) * e;
+      st->hbm_bytes += 20 * n;  // update pass over rank / delta / residual
+    }
+    cur ^= 1;
+    if (w.h_aux[0] == 0) break;
+  }
+  if (st) {
+    st->ms = (now_ns() - t0) * 1e-6;
+    st->expand_ms = expand_ms;
+    st->reached = n;
+  }
+  if (out && n) HIPC(hipMemcpy(out, rank, n * 4, hipMemcpyDeviceToHost));
+}
+
+}  // namespace thv

@@ -1,0 +1,186 @@
+// expand.hip -- edge-balanced frontier expansion, the hot kernel.
+//
+// Replaces the reference's warp-per-vertex edge kernels
+// (BFS32_Static/Demand/Filter_Kernel src/bfs/bfs_kernels.cu:4-76, 111-154,
+// 196-287; CC32_* src/cc/cc_kernels.cu; SSSP32_* src/sssp/sssp_kernels.cu;
+// PR32_* src/pr/pr_kernels.cu) with one kernel per algorithm that walks the
+// level's work space -- the concatenated edge lists of the queued vertices --
+// in chunks of C = 256 * EPT edges.  Host-resident chunks come first so
+// their zero-copy reads are in flight while HBM chunks are processed.
+//
+// Per chunk (one 256-thread block):
+//   1. the chunk-start table names the first queue entry; entries whose
+//      lists intersect the chunk are staged in LDS (at most C + 1), and the
+//      chunk-relative start of each is marked in an LDS bit mask;
+//   2. one wave turns the mask into per-word prefix popcounts;
+//   3. every thread takes EPT edges spaced 256 apart: its entry is the rank
+//      of the last mark at or before its position (popcount), its record
+//      address is entry_base + edge_index * record_bytes; all EPT loads are
+//      issued before any is consumed, then each edge is relaxed.
+// Hubs are therefore split across many blocks and low-degree vertices share
+// a wave, with consecutive lanes reading consecutive addresses whenever the
+// queued vertices are adjacent (coalesced 256-byte wave reads).
+//
+// Relaxation rules (identical fixpoints to the reference):
+//   BFS  : first visit wins (atomicOr on the visited bitmap), value = level
+//          (bfs_kernels.cu:29-44)
+//   CC   : atomicMin(val[w], val[v]) (cc_kernels.cu:50-66)
+//   SSSP : atomicMin(val[w], val[v] + weight) (sssp_kernels.cu:124-141)
+//   PR   : push atomicAdd(res[w], delta[v]) (pr_kernels.cu:539-542);
+//          pull sum[v] += contrib[u] over in-edges (pr_kernels.cu:45-51)
+#include "internal.hpp"
+#include "scan.hpp"
+
+namespace thv {
+
+constexpr int EBS = 256;
+
+template <int ALGO, int REC, int EPT>
+__global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
+  constexpr uint32_t C = EBS * EPT;
+  constexpr uint32_t NW = C / 32;
+  __shared__ uint32_t s_mask[NW];
+  __shared__ uint32_t s_pre[NW];
+  __shared__ unsigned long long s_base[C + 1];
+  __shared__ uint32_t s_u[C + 1];  // CC/SSSP: source value; PR pull: vertex id
+  __shared__ float s_f[C + 1];     // PR push: delta; PR pull: partial sum
+
+  const unsigned long long Wh = a.ctr->qw[Q_HOST], Wd = a.ctr->qw[Q_DEV];
+  const unsigned long long nqh = a.ctr->qn[Q_HOST], nqd = a.ctr->qn[Q_DEV];
+  const uint32_t Hc = (uint32_t)((Wh + C - 1) / C), Dc = (uint32_t)((Wd + C - 1) / C);
+  const uint32_t tid = threadIdx.x;
+
+  for (uint32_t c = blockIdx.x; c < Hc + Dc; c += gridDim.x) {
+    const int q = c < Hc ? Q_HOST : Q_DEV;
+    const uint32_t cc = q == Q_HOST ? c : c - Hc;
+    const uint32_t nch = q == Q_HOST ? Hc : Dc;
+    const unsigned long long W = q == Q_HOST ? Wh : Wd;
+    const unsigned long long nq = q == Q_HOST ? nqh : nqd;
+    const uint32_t *__restrict__ qv = a.qv[q];
+    const unsigned long long *__restrict__ qe = a.qe[q];
+    const unsigned long long e0 = (unsigned long long)cc * C;
+    const uint32_t len = (uint32_t)min((unsigned long long)C, W - e0);
+    const uint32_t i0 = a.qc[q][cc];
+    const uint32_t i1 = cc + 1 < nch ? a.qc[q][cc + 1] : (uint32_t)(nq - 1);
+    const uint32_t nent = i1 - i0 + 1;
+
+    for (uint32_t t = tid; t < NW; t += EBS) s_mask[t] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < nent; t += EBS) {
+      const uint32_t v = qv[i0 + t];
+      const unsigned long long b = qe[i0 + t];
+      const unsigned long long s = b > e0 ? b - e0 : 0;
+      if (s < C) {
+        atomicOr(&s_mask[s >> 5], 1u << (s & 31));
+        s_base[t] = a.eptr[v] - b * REC;
+        if (ALGO == A_CC || ALGO == A_SSSP) s_u[t] = a.val[v];
+        if (ALGO == A_PR_PUSH) s_f[t] = a.pr_src[v];
+        if (ALGO == A_PR_PULL) {
+          s_u[t] = v;
+          s_f[t] = 0.0f;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {
+      uint32_t pc = tid < NW ? __popc(s_mask[tid]) : 0u;
+      uint32_t inc = wave_incl_scan(pc);
+      if (tid < NW) s_pre[tid] = inc - pc;
+    }
+    __syncthreads();
+
+    uint32_t dst[EPT], wt[EPT], ent[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      const uint32_t l = tid + j * EBS;
+      dst[j] = 0xFFFFFFFFu;
+      wt[j] = 0;
+      ent[j] = 0;
+      if (l < len) {
+        const uint32_t wd = l >> 5;
+        const uint32_t k = s_pre[wd] + __popc(s_mask[wd] & (0xFFFFFFFFu >> (31 - (l & 31)))) - 1;
+        ent[j] = k;
+        const unsigned long long addr = s_base[k] + (e0 + l) * REC;
+        if (REC == 8) {
+          const uint2 r = *reinterpret_cast<const uint2 *>(addr);
+          dst[j] = r.x;
+          wt[j] = r.y;
+        } else {
+          dst[j] = *reinterpret_cast<const uint32_t *>(addr);
+        }
+      }
+    }
+
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      const uint32_t w = dst[j];
+      if (w == 0xFFFFFFFFu) continue;
+      const uint32_t wd = w >> 5, bit = 1u << (w & 31);
+      if (ALGO == A_BFS) {
+        if (!(a.vis[wd] & bit)) {
+          const uint32_t old = atomicOr(&a.vis[wd], bit);
+          if (!(old & bit)) {
+            a.val[w] = a.level_value;
+            atomicOr(&a.next[wd], bit);
+          }
+        }
+      } else if (ALGO == A_CC || ALGO == A_SSSP) {
+        const unsigned long long nv64 =
+            (unsigned long long)s_u[ent[j]] + (ALGO == A_SSSP ? wt[j] : 0u);
+        if (nv64 < a.val[w]) {
+          const uint32_t nv = (uint32_t)nv64;
+          const uint32_t old = atomicMin(&a.val[w], nv);
+          if (nv < old && !(a.next[wd] & bit)) atomicOr(&a.next[wd], bit);
+        }
+      } else if (ALGO == A_PR_PUSH) {
+        unsafeAtomicAdd(&a.pr_dst[w], s_f[ent[j]]);
+      } else {  // A_PR_PULL: w is an in-neighbour of the entry's vertex
+        atomicAdd(&s_f[ent[j]], a.pr_src[w]);
+      }
+    }
+    if (ALGO == A_PR_PULL) {
+      __syncthreads();
+      for (uint32_t t = tid; t < nent; t += EBS) {
+        const unsigned long long b = qe[i0 + t];
+        if ((b > e0 ? b - e0 : 0) < C) unsafeAtomicAdd(&a.pr_dst[s_u[t]], s_f[t]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int ALGO, int REC>
+static void launch_t(Graph &g, const ExpandArgs &args, hipStream_t s) {
+  const int grid = g.shape.blocks_per_cu * g.cus;
+  switch (g.shape.ept) {
+    case 4:
+      hipLaunchKernelGGL((k_expand<ALGO, REC, 4>), dim3(grid), dim3(EBS), 0, s, args);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_expand<ALGO, REC, 8>), dim3(grid), dim3(EBS), 0, s, args);
+      break;
+    default:
+      fail(THV_EINVAL, "unsupported edges-per-thread");
+  }
+}
+
+void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
+  const bool r8 = g.rec == 8;
+  switch (a) {
+    case A_BFS: r8 ? launch_t<A_BFS, 8>(g, args, s) : launch_t<A_BFS, 4>(g, args, s); break;
+    case A_CC: r8 ? launch_t<A_CC, 8>(g, args, s) : launch_t<A_CC, 4>(g, args, s); break;
+    case A_SSSP:
+      if (!r8) fail(THV_ENOTSUP, "SSSP needs 8-byte {dst, weight} records");
+      launch_t<A_SSSP, 8>(g, args, s);
+      break;
+    case A_PR_PUSH:
+      r8 ? launch_t<A_PR_PUSH, 8>(g, args, s) : launch_t<A_PR_PUSH, 4>(g, args, s);
+      break;
+    case A_PR_PULL:
+      r8 ? launch_t<A_PR_PULL, 8>(g, args, s) : launch_t<A_PR_PULL, 4>(g, args, s);
+      break;
+  }
+  HIPC(hipGetLastError());
+}
+
+}  // namespace thv

@@ -1,0 +1,262 @@
+// graph.cpp -- graph loading, the split edge store and the per-graph
+// workspace.
+//
+// Reference counterparts: CSR<uint32>::ReadInputFile (src/graph.cuh:641-815),
+// SetPartitionsConfig (graph.cuh:270-304), SetNumStaticEdges
+// (graph.cuh:306-341) and the allocations of InitData (graph.cuh:344-638).
+// Differences by design: the whole edge array lives in one hipHostMalloc'd
+// mapped buffer (no NUMA replicas), the HBM share is chosen by partition
+// average degree under a byte budget instead of as a vertex prefix of 90%
+// of free memory, and it is filled once at configuration time instead of
+// lazily by the filter path.
+#include <algorithm>
+#include <cstdio>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace thv {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &m) { g_last_error = m; }
+const char *last_error() { return g_last_error.c_str(); }
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+void hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+           hipGetErrorString(e), what, file, line);
+  int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? THV_ENOMEM
+                                                                          : THV_EHIP;
+  throw Error(code, buf);
+}
+
+uint64_t now_ns() {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+void DevMem::alloc(size_t b) {
+  release();
+  if (b == 0) b = 8;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    fail(THV_ENOMEM, "hipMalloc of " + std::to_string(b) + " bytes failed");
+  }
+  bytes = b;
+}
+
+void DevMem::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// Non-coherent (coarse-grained) mapped memory lets the GPU cache the
+// read-only edge records in L2; THV_HOST_COHERENT=1 selects fine-grained.
+static unsigned host_flags() {
+  const char *e = getenv("THV_HOST_COHERENT");
+  if (e && e[0] == '1') return hipHostMallocMapped | hipHostMallocCoherent;
+  return hipHostMallocMapped | hipHostMallocNonCoherent;
+}
+
+void HostPinned::alloc(size_t b) {
+  release();
+  if (b == 0) b = 8;
+  hipError_t e = hipHostMalloc(&p, b, host_flags());
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    fail(THV_ENOMEM, "hipHostMalloc of " + std::to_string(b) + " bytes failed");
+  }
+  HIPC(hipHostGetDevicePointer(&dev, p, 0));
+  bytes = b;
+}
+
+void HostPinned::release() {
+  if (p) (void)hipHostFree(p);
+  p = dev = nullptr;
+  bytes = 0;
+}
+
+Workspace::~Workspace() {
+  if (h_ctr) (void)hipHostFree(h_ctr);
+  if (h_aux) (void)hipHostFree(h_aux);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+Graph::~Graph() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+void graph_init_device(Graph &g) {
+  HIPC(hipGetDevice(&g.device));
+  hipDeviceProp_t prop;
+  HIPC(hipGetDeviceProperties(&prop, g.device));
+  g.cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  HIPC(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+}
+
+// Device degree tables + default store (everything in HBM).
+void graph_build_tables(Graph &g) {
+  std::vector<uint32_t> deg(g.n);
+  for (uint64_t v = 0; v < g.n; v++) {
+    uint64_t d = g.h_off[v + 1] - g.h_off[v];
+    if (d > 0xFFFFFFFFull) fail(THV_ENOTSUP, "vertex degree exceeds 2^32-1");
+    deg[v] = (uint32_t)d;
+  }
+  g.d_deg.alloc(g.n * 4);
+  if (g.n) HIPC(hipMemcpy(g.d_deg.p, deg.data(), g.n * 4, hipMemcpyHostToDevice));
+  store_configure(g, UINT64_MAX, 1u << 16);
+}
+
+// Greedy contiguous vertex ranges of at most part_edges edges; a vertex
+// larger than that gets a partition of its own (graph.cuh:270-304).
+static void make_partitions(Graph &g, uint32_t part_edges, std::vector<uint32_t> &pv) {
+  pv.clear();
+  pv.push_back(0);
+  uint64_t cur = 0;
+  for (uint64_t v = 0; v < g.n; v++) {
+    uint64_t d = g.h_off[v + 1] - g.h_off[v];
+    if (cur + d <= part_edges || v == pv.back()) {
+      cur += d;
+    } else {
+      pv.push_back((uint32_t)v);
+      cur = d;
+    }
+  }
+  pv.push_back((uint32_t)g.n);
+  if (g.n == 0) pv.assign({0, 0});
+}
+
+void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
+  if (part_edges == 0) fail(THV_EINVAL, "partition_edges must be > 0");
+  if (g.n >= 0xFFFFFFFFull) fail(THV_ENOTSUP, "graphs need fewer than 2^32-1 vertices");
+  Store &s = g.store;
+  HIPC(hipStreamSynchronize(g.stream));
+  s.part_edges = part_edges;
+  s.budget = budget;
+  make_partitions(g, part_edges, s.pv);
+  s.P = (uint32_t)(s.pv.size() - 1);
+  const uint64_t rec = (uint64_t)g.rec;
+
+  // Residency: partitions ranked by average degree (edges per vertex),
+  // densest first, taken whole while the byte budget allows.
+  std::vector<uint32_t> order(s.P);
+  std::iota(order.begin(), order.end(), 0u);
+  auto pedges = [&](uint32_t p) { return g.h_off[s.pv[p + 1]] - g.h_off[s.pv[p]]; };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    uint64_t ea = pedges(a), eb = pedges(b);
+    uint64_t va = s.pv[a + 1] - s.pv[a], vb = s.pv[b + 1] - s.pv[b];
+    // ea/va > eb/vb  <=>  ea*vb > eb*va  (128-bit products)
+    unsigned __int128 l = (unsigned __int128)ea * vb, r = (unsigned __int128)eb * va;
+    return l > r;
+  });
+  s.pres.assign(s.P, 0);
+  uint64_t used = 0;
+  s.dev_edges = 0;
+  s.Pdev = 0;
+  for (uint32_t p : order) {
+    uint64_t e = pedges(p);
+    if (e == 0) {  // nothing to store; count it as device resident
+      s.pres[p] = 1;
+      s.Pdev++;
+      continue;
+    }
+    if (budget != UINT64_MAX && used + e * rec > budget) continue;
+    s.pres[p] = 1;
+    used += e * rec;
+    s.dev_edges += e;
+    s.Pdev++;
+  }
+
+  // Slab: resident partitions packed in vertex order; contiguous runs of
+  // resident partitions are copied with one transfer each.
+  s.slab.alloc(s.dev_edges * rec);
+  std::vector<uint64_t> slab_off(s.P, 0);
+  uint64_t pos = 0;
+  for (uint32_t p = 0; p < s.P;) {
+    if (!s.pres[p] || pedges(p) == 0) {
+      p++;
+      continue;
+    }
+    uint32_t q = p;
+    uint64_t run_start_edge = g.h_off[s.pv[p]];
+    uint64_t run_pos = pos;
+    while (q < s.P && s.pres[q]) {
+      slab_off[q] = pos;
+      pos += pedges(q);
+      q++;
+    }
+    uint64_t run_edges = pos - run_pos;
+    if (run_edges)
+      HIPC(hipMemcpy(s.slab.as<char>() + run_pos * rec,
+                     (const char *)g.h_edges.p + run_start_edge * rec, run_edges * rec,
+                     hipMemcpyHostToDevice));
+    p = q;
+  }
+
+  // Per-vertex first-record address.
+  std::vector<uint64_t> eptr(g.n);
+  const uint64_t slab_base = (uint64_t)(uintptr_t)s.slab.p;
+  const uint64_t host_base = (uint64_t)(uintptr_t)g.h_edges.dev;
+  for (uint32_t p = 0; p < s.P; p++) {
+    uint64_t pe0 = g.h_off[s.pv[p]];
+    for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) {
+      if (s.pres[p])
+        eptr[v] = slab_base + (slab_off[p] + (g.h_off[v] - pe0)) * rec;
+      else
+        eptr[v] = host_base + g.h_off[v] * rec;
+    }
+  }
+  s.d_eptr.alloc(g.n * 8);
+  if (g.n) HIPC(hipMemcpy(s.d_eptr.p, eptr.data(), g.n * 8, hipMemcpyHostToDevice));
+  s.d_pv.alloc((s.P + 1) * 4);
+  HIPC(hipMemcpy(s.d_pv.p, s.pv.data(), (s.P + 1) * 4, hipMemcpyHostToDevice));
+  s.d_pres.alloc(s.P ? s.P : 1);
+  if (s.P) HIPC(hipMemcpy(s.d_pres.p, s.pres.data(), s.P, hipMemcpyHostToDevice));
+}
+
+void workspace_ensure(Graph &g) {
+  Workspace &w = g.ws;
+  const uint32_t C = 256u * (uint32_t)g.shape.ept;
+  if (w.P != g.store.P || !w.pcnt.p) {
+    const uint64_t P = g.store.P + 1;
+    w.pcnt.alloc(P * 4);
+    w.pdeg.alloc(P * 8);
+    w.pbc.alloc(P * 4);
+    w.pbe.alloc(P * 8);
+    w.P = g.store.P;
+  }
+  if (w.n == g.n && w.m == g.m && w.C == C && w.val.p) return;
+  const uint64_t words = g.n / 32 + 2;
+  for (auto &f : w.front) f.alloc(words * 4);
+  w.vis.alloc(words * 4);
+  for (int q = 0; q < 2; q++) {
+    w.qv[q].alloc((g.n + 1) * 4);
+    w.qe[q].alloc((g.n + 1) * 8);
+    w.qc[q].alloc((g.m / C + 2) * 4);
+  }
+  w.ctr.alloc(2 * sizeof(LevelCtr));
+  w.aux.alloc(8 * sizeof(unsigned long long));
+  w.val.alloc((g.n + 1) * 4);
+  if (!w.h_ctr) HIPC(hipHostMalloc((void **)&w.h_ctr, 2 * sizeof(LevelCtr), hipHostMallocDefault));
+  if (!w.h_aux)
+    HIPC(hipHostMalloc((void **)&w.h_aux, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+  for (auto &e : w.ev)
+    if (!e) HIPC(hipEventCreate(&e));
+  w.n = g.n;
+  w.m = g.m;
+  w.C = C;
+}
+
+}  // namespace thv

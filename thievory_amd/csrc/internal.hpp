@@ -1,0 +1,203 @@
+// internal.hpp -- shared definitions of the MI355X graph engine.
+//
+// Layout in HBM (per graph):
+//   deg[n]    u32  edge-list length of every vertex (out-degree for a CSR,
+//                  in-degree for a CSC)
+//   eptr[n]   u64  address of the vertex's first edge record, either in the
+//                  device slab (partition resident in HBM) or in the mapped
+//                  pinned host copy (read zero-copy over the host link)
+//   pv[P+1]   u32  first vertex of every partition; pres[P] u8 residency
+//   slab           the HBM-resident partitions' edge records, packed
+// plus the per-algorithm workspace (frontier bitmaps, queues, values).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/thv.h"
+
+namespace thv {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string &msg);
+void hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define HIPC(x)                                                   \
+  do {                                                            \
+    hipError_t _e = (x);                                          \
+    if (_e != hipSuccess) ::thv::hip_fail(_e, #x, __FILE__, __LINE__); \
+  } while (0)
+
+// ---- device memory helpers ------------------------------------------------
+
+struct DevMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  ~DevMem() { release(); }
+  void alloc(size_t b);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct HostPinned {  // hipHostMalloc'd, mapped into the GPU address space
+  void *p = nullptr;
+  void *dev = nullptr;
+  size_t bytes = 0;
+  HostPinned() = default;
+  HostPinned(const HostPinned &) = delete;
+  HostPinned &operator=(const HostPinned &) = delete;
+  ~HostPinned() { release(); }
+  void alloc(size_t b);
+  void release();
+};
+
+// ---- store -------------------------------------------------------------------
+
+struct Store {
+  uint32_t part_edges = 1u << 16;
+  uint64_t budget = UINT64_MAX;      // device edge budget in bytes
+  uint32_t P = 0;                    // partitions
+  uint32_t Pdev = 0;                 // partitions resident in HBM
+  uint64_t dev_edges = 0;            // edges resident in HBM
+  std::vector<uint32_t> pv;          // P+1 first vertex of each partition
+  std::vector<uint8_t> pres;         // P residency (1 = HBM)
+  DevMem slab;                       // resident records
+  DevMem d_pv, d_pres, d_eptr;
+};
+
+// ---- workspace (per graph, allocated on first use) --------------------------
+
+enum { Q_HOST = 0, Q_DEV = 1 };
+
+// Counters written by the compaction kernel for one level.
+struct LevelCtr {
+  unsigned long long qn[2];   // queue entries (host, device)
+  unsigned long long qw[2];   // queue edges
+  unsigned int touched[2];    // partitions with >=1 active vertex
+  unsigned int pad[2];
+};
+
+struct Workspace {
+  uint64_t n = 0, m = 0;
+  uint32_t C = 0;             // edges per expansion chunk
+  DevMem front[2];            // frontier bitmaps (n bits, word padded)
+  DevMem vis;                 // BFS visited bitmap
+  DevMem qv[2], qe[2], qc[2]; // queue vertex ids, edge prefixes, chunk starts
+  DevMem ctr;                 // LevelCtr[2]
+  DevMem pcnt, pdeg, pbc, pbe; // per partition: active count / edges, bases
+  uint32_t P = 0;
+  DevMem val;                 // u32 values (BFS / CC / SSSP)
+  DevMem f0, f1, f2;          // PR arrays (fp32)
+  DevMem aux;                 // misc device counters (8 x u64)
+  LevelCtr *h_ctr = nullptr;  // pinned readback
+  unsigned long long *h_aux = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~Workspace();
+};
+
+// ---- launch shape -------------------------------------------------------------
+
+struct Shape {
+  int blocks_per_cu = 4;      // expansion grid = blocks_per_cu * CUs
+  int ept = 8;                // edges per thread per chunk (chunk = 256*ept)
+};
+
+// ---- graph -------------------------------------------------------------------
+
+struct Graph {
+  uint64_t n = 0, m = 0;
+  int rec = 4;                // edge record bytes (4 = dst, 8 = {dst, w})
+  bool csc = false;           // edge lists are in-lists (PageRank pull)
+  std::vector<uint64_t> h_off;  // n+1
+  HostPinned h_edges;         // m records, mapped
+  DevMem d_deg;               // n u32
+  DevMem d_outdeg;            // n u32 (CSC only)
+  Store store;
+  Workspace ws;
+  Shape shape;
+  bool tuned = false;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int cus = 256;
+  std::mutex mu;              // serialises algorithm calls on one handle
+  ~Graph();
+};
+
+// graph.cpp
+void graph_init_device(Graph &g);
+void graph_build_tables(Graph &g);
+void store_configure(Graph &g, uint64_t budget, uint32_t part_edges);
+void workspace_ensure(Graph &g);
+uint64_t now_ns();
+
+// kernels (frontier.hip / expand.hip / algos.hip)
+enum Algo { A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4 };
+
+struct ExpandArgs {
+  const uint32_t *qv[2];
+  const unsigned long long *qe[2];
+  const uint32_t *qc[2];
+  const LevelCtr *ctr;
+  const uint64_t *eptr;
+  uint32_t *val;            // BFS/CC/SSSP values
+  uint32_t *vis;            // BFS visited bitmap
+  uint32_t *next;           // next frontier bitmap
+  float *pr_src;            // PR push: delta; PR pull: contribution
+  float *pr_dst;            // PR push: residual; PR pull: sum
+  uint32_t level_value;     // BFS: value given to discovered vertices
+  uint32_t C;
+  int rec;
+};
+
+void launch_compact(Graph &g, const uint32_t *front, LevelCtr *ctr,
+                    hipStream_t s);
+void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);
+
+// algos.hip
+void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
+void run_cc(Graph &g, uint32_t *out, thv_stats *st);
+void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
+void run_pr(Graph &g, int mode, float damping, float tol, int max_iter,
+            float *out, thv_stats *st);
+void launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+
+// rmat.hip
+Graph *rmat_generate(int scale, int edge_factor, double a, double b, double c,
+                     uint64_t seed, bool symmetric, bool weights,
+                     bool transpose);
+
+// batch.hip
+void run_batch(Graph &g, int algo, const uint32_t *srcs, uint32_t k,
+               uint64_t *hash, uint64_t *reached, uint32_t *depth,
+               uint32_t *full, thv_stats *st);
+
+// autotune.cpp
+void autotune(Graph &g, const char *cache_path, int *chosen);
+
+// shared hash (identical to the oracle's): SplitMix64 output function
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+}  // namespace thv
