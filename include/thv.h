@@ -65,7 +65,7 @@ typedef struct {
   uint32_t iterations;      /* levels / rounds until the frontier emptied */
   uint32_t host_partitions_touched; /* host partitions read, summed      */
   uint32_t host_partitions_skipped; /* host partitions skipped, summed    */
-  uint32_t reserved;
+  uint32_t expand_launches;         /* edge-expansion kernel launches      */
   double ms;                /* first launch -> empty frontier (wall)      */
   double expand_ms;         /* time inside edge-expansion kernels (events)*/
 } thv_stats;

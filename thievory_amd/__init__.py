@@ -43,11 +43,11 @@ class Stats(C.Structure):
                 ("host_bytes", C.c_uint64), ("moved_bytes", C.c_uint64),
                 ("reached", C.c_uint64), ("iterations", C.c_uint32),
                 ("host_partitions_touched", C.c_uint32),
-                ("host_partitions_skipped", C.c_uint32), ("reserved", C.c_uint32),
+                ("host_partitions_skipped", C.c_uint32), ("expand_launches", C.c_uint32),
                 ("ms", C.c_double), ("expand_ms", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ServeCfg(C.Structure):
@@ -127,6 +127,14 @@ def check(rc):
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(_vp)
+
+
+def _out(out, n, dtype):
+    """out=None allocates a host array; out=False keeps the result in HBM
+    only (no readback: the call then measures just the traversal)."""
+    if out is False:
+        return None
+    return np.zeros(n, dtype) if out is None else out
 
 
 def device_info():
@@ -212,25 +220,25 @@ class Graph:
 
     # ---- algorithms -------------------------------------------------------------
     def bfs(self, src, out=None):
-        out = np.zeros(self.n, np.uint32) if out is None else out
+        out = _out(out, self.n, np.uint32)
         st = Stats()
         check(lib().thv_bfs(self._h, src, _p(out), C.byref(st)))
         return out, st
 
     def cc(self, out=None):
-        out = np.zeros(self.n, np.uint32) if out is None else out
+        out = _out(out, self.n, np.uint32)
         st = Stats()
         check(lib().thv_cc(self._h, _p(out), C.byref(st)))
         return out, st
 
     def sssp(self, src, out=None):
-        out = np.zeros(self.n, np.uint32) if out is None else out
+        out = _out(out, self.n, np.uint32)
         st = Stats()
         check(lib().thv_sssp(self._h, src, _p(out), C.byref(st)))
         return out, st
 
     def pr(self, mode="push", damping=0.85, tol=1e-7, max_iter=100, out=None):
-        out = np.zeros(self.n, np.float32) if out is None else out
+        out = _out(out, self.n, np.float32)
         st = Stats()
         check(lib().thv_pr(self._h, PR_MODE[mode], damping, tol, max_iter, _p(out),
                            C.byref(st)))

@@ -186,6 +186,7 @@ static void value_loop(Graph &g, Algo algo, thv_stats *st) {
   LevelCtr *d_ctr = w.ctr.as<LevelCtr>();
   int cur = 0;
   double expand_ms = 0;
+  uint32_t launches = 0;
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
@@ -200,16 +201,20 @@ static void value_loop(Graph &g, Algo algo, thv_stats *st) {
     HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const LevelCtr &h = w.h_ctr[L & 1];
-    if (h.qn[0] + h.qn[1] == 0) break;
+    // every launch is counted, the last one (empty frontier, exits at once)
+    // included, so launches and time match a kernel trace of the same run
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
     expand_ms += ms;
+    launches++;
+    if (h.qn[0] + h.qn[1] == 0) break;
     stats_level(g, st, h);
     cur ^= 1;
   }
   if (st) {
     st->ms = (now_ns() - t0) * 1e-6;
     st->expand_ms = expand_ms;
+    st->expand_launches = launches;
   }
 }
 
@@ -358,6 +363,7 @@ void run_pr(Graph &g, int mode, float alpha, float tol, int max_iter, float *out
       st->hbm_bytes += 20 * n;  // update pass over rank / delta / residual
     }
     cur ^= 1;
+    if (st) st->expand_launches++;
     if (w.h_aux[0] == 0) break;
   }
   if (st) {
