@@ -197,6 +197,8 @@ const char *thv_last_error(void);
 const char *thv_version(void);
 int thv_device_info(char *name, size_t name_len, int *cus,
                     uint64_t *hbm_bytes);
+/* Free and total device memory (hipMemGetInfo). */
+int thv_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,147 @@
+"""GPU tests of batched multi-source queries, the continuous-batching server
+and the autotuner.  Every lane's digest, reached count and depth must equal
+the oracle's single-source answer (bit-exact); served results must equal
+the same query run alone."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import thievory_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _expect(og, algo, src):
+    val = O.bfs(og, src) if algo == "bfs" else O.sssp(og, src)
+    h, r, d = O.hash_values(val, og.n + 1)
+    return val, h, r, d
+
+
+@pytest.fixture(scope="module")
+def g14():
+    og = O.rmat(14, 16, seed=21, weights=True)
+    g = T.Graph.rmat(14, 16, seed=21, weights=True)
+    return g, og
+
+
+@pytest.mark.parametrize("algo", ["bfs", "sssp"])
+@pytest.mark.parametrize("frac", [1.0, 0.25])
+def test_batch_64_matches_single_source(g14, algo, frac):
+    g, og = g14
+    _say("case: batch", algo, "budget", frac)
+    g.store_config(int(frac * g.m * g.edge_bytes), 1 << 12)
+    rng = np.random.default_rng(5)
+    srcs = rng.integers(0, og.n, 64).astype(np.uint32)
+    srcs[3] = srcs[7]  # duplicate sources are independent lanes
+    out = g.batch(algo, srcs, full=True)
+    for i, s in enumerate(srcs):
+        val, h, r, d = _expect(og, algo, int(s))
+        assert np.array_equal(out["full"][i], val), (i, s)
+        assert (out["hash"][i], out["reached"][i], out["depth"][i]) == (h, r, d), (i, s)
+
+
+@pytest.mark.parametrize("k", [1, 13])
+def test_batch_small_k(g14, k):
+    g, og = g14
+    g.store_config(None, 1 << 16)
+    srcs = np.arange(k, dtype=np.uint32) * 97 % og.n
+    for algo in ("bfs", "sssp"):
+        out = g.batch(algo, srcs)
+        for i, s in enumerate(srcs):
+            _, h, r, d = _expect(og, algo, int(s))
+            assert (out["hash"][i], out["reached"][i], out["depth"][i]) == (h, r, d)
+
+
+def test_batch_rejects_bad_input(g14):
+    g, og = g14
+    with pytest.raises(T.ThvError):
+        g.batch("bfs", np.zeros(65, np.uint32))
+    with pytest.raises(T.ThvError):
+        g.batch("bfs", np.array([og.n], np.uint32))
+    with pytest.raises(T.ThvError):
+        g.batch("cc", np.array([0], np.uint32))
+
+
+def test_serving_alone_equals_mix(g14):
+    """Queries arrive while others run (8 lanes per engine, 300 queries):
+    each result equals the query run alone -- on the oracle and through the
+    plain single-source GPU path."""
+    g, og = g14
+    _say("case: serving mix")
+    g.store_config(int(0.5 * g.m * 8), 1 << 12)
+    srv = T.Server(g, max_lanes=8)
+    rng = np.random.default_rng(9)
+    qs = [("bfs" if rng.random() < 0.5 else "sssp", int(rng.integers(0, og.n)))
+          for _ in range(300)]
+    tickets = [srv.submit(a, s) for a, s in qs]
+    srv.drain()
+    results = [srv.poll(t) for t in tickets]
+    srv.stop()
+    for (a, s), r in zip(qs, results):
+        assert r["status"] == 0
+        _, h, rc, d = _expect(og, a, s)
+        assert (r["hash"], r["reached"], r["depth"]) == (h, rc, d), (a, s)
+        assert r["submit_ms"] <= r["start_ms"] <= r["done_ms"]
+    # alone on the GPU: plain single-source kernels
+    for a, s in qs[:20]:
+        val = g.bfs(s)[0] if a == "bfs" else g.sssp(s)[0]
+        h, rc, d = O.hash_values(val, og.n + 1)
+        r = results[qs.index((a, s))]
+        assert (r["hash"], r["reached"], r["depth"]) == (h, rc, d)
+
+
+def test_serving_keep_full_and_reuse():
+    og = O.rmat(11, 8, seed=4, weights=True)
+    g = T.Graph.from_arrays(og.off, og.adj, og.w)
+    srv = T.Server(g, max_lanes=4, keep_full=True)
+    srcs = [0, 5, 17, 99, 1000, 2047, 3, 3, 512, 64]
+    tk = [(a, s, srv.submit(a, s)) for s in srcs for a in ("bfs", "sssp")]
+    srv.drain()
+    for a, s, t in tk:
+        full = np.zeros(og.n, np.uint32)
+        r = srv.poll(t, full)
+        val = O.bfs(og, s) if a == "bfs" else O.sssp(og, s)
+        assert r["status"] == 0 and np.array_equal(full, val), (a, s)
+    srv.stop()
+
+
+def test_serving_errors():
+    og = O.rmat(10, 8, seed=4)
+    g = T.Graph.from_arrays(og.off, og.adj)
+    with pytest.raises(T.ThvError):
+        T.Server(g, scratch_bytes=1024)
+    srv = T.Server(g, max_lanes=2)
+    with pytest.raises(T.ThvError) as e:
+        srv.submit("sssp", 0)          # unweighted graph: no SSSP engine
+    assert e.value.code == -7
+    with pytest.raises(T.ThvError):
+        srv.submit("bfs", og.n)
+    with pytest.raises(T.ThvError) as e:
+        g.store_config(None, 128)      # store is pinned while serving
+    assert e.value.code == -6
+    t = srv.submit("bfs", 1)
+    srv.drain()
+    assert srv.poll(t)["status"] == 0
+    srv.stop()
+    g.store_config(None, 128)
+
+
+def test_autotune_keeps_results_exact(tmp_path, g14):
+    g, og = g14
+    g.store_config(int(0.25 * g.m * 8), 1 << 12)
+    cache = str(tmp_path / "tune.txt")
+    ch = g.autotune(cache)
+    assert ch["blocks_per_cu"] in (2, 4, 8) and ch["edges_per_thread"] in (4, 8)
+    assert open(cache).read().count("|") == 1
+    assert g.autotune(cache) == ch          # second call reads the cache
+    assert open(cache).read().count("|") == 1
+    assert np.array_equal(g.bfs(1)[0], O.bfs(og, 1))
+    assert np.array_equal(g.sssp(1)[0], O.sssp(og, 1))
+    out = g.batch("bfs", np.arange(64, dtype=np.uint32))
+    for i in range(64):
+        _, h, r, d = _expect(og, "bfs", i)
+        assert (out["hash"][i], out["reached"][i], out["depth"][i]) == (h, r, d)

@@ -109,6 +109,7 @@ def lib():
         "thv_version": (C.c_char_p, []),
         "thv_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
                                       C.POINTER(C.c_uint64)]),
+        "thv_device_memory": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -142,6 +143,12 @@ def device_info():
     cus, hbm = C.c_int(), C.c_uint64()
     check(lib().thv_device_info(name, 64, C.byref(cus), C.byref(hbm)))
     return {"arch": name.value.decode(), "cus": cus.value, "hbm_bytes": hbm.value}
+
+
+def device_memory():
+    fr, tot = C.c_uint64(), C.c_uint64()
+    check(lib().thv_device_memory(C.byref(fr), C.byref(tot)))
+    return {"free": fr.value, "total": tot.value}
 
 
 class Graph:

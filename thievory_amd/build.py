@@ -32,7 +32,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Wall", "-Wno-unused-result", "-I", INCLUDE]
 
 LIB_SOURCES = ["graph.cpp", "io.cpp", "abi.cpp", "frontier.hip", "expand.hip",
-               "algos.hip", "rmat.hip", "batch.hip", "serve.cpp"]
+               "algos.hip", "rmat.hip", "batch.hip", "serve.cpp", "autotune.cpp"]
 CLI_SOURCES = ["cli.cpp"]
 
 
@@ -57,8 +57,11 @@ def _compile(src, force):
     obj = _obj(src)
     if not force and not _stale(obj, [path] + _headers()):
         return obj, None
-    lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [HIPCC] + COMMON + lang + ["-c", path, "-o", obj]
+    if src in CLI_SOURCES:  # plain C++ against the C ABI only
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-c", path, "-o", obj]
+    else:
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        cmd = [HIPCC] + COMMON + lang + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         return obj, f"{' '.join(cmd)}\n{r.stdout}\n{r.stderr}"

@@ -102,6 +102,7 @@ int thv_store_config(thv_graph *gh, uint64_t budget, uint32_t partition_edges) {
     NEED(gh);
     Graph *g = reinterpret_cast<Graph *>(gh);
     std::lock_guard<std::mutex> lk(g->mu);
+    if (g->servers.load()) fail(THV_EBUSY, "a server is running on this graph");
     store_configure(*g, budget, partition_edges);
   });
 }
@@ -110,7 +111,11 @@ int thv_store_info(const thv_graph *gh, uint32_t *partitions, uint32_t *device_p
                    uint64_t *device_edges) {
   return guarded([&] {
     NEED(gh);
-    const Graph *g = reinterpret_cast<const Graph *>(gh);
+    Graph *g = const_cast<Graph *>(reinterpret_cast<const Graph *>(gh));
+    {
+      std::lock_guard<std::mutex> lk(g->mu);
+      store_ensure(*g);
+    }
     if (partitions) *partitions = g->store.P;
     if (device_partitions) *device_partitions = g->store.Pdev;
     if (device_edges) *device_edges = g->store.dev_edges;
@@ -193,6 +198,15 @@ int thv_device_info(char *name, size_t name_len, int *cus, uint64_t *hbm_bytes) 
     }
     if (cus) *cus = p.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+  });
+}
+
+int thv_device_memory(uint64_t *free_bytes, uint64_t *total_bytes) {
+  return guarded([&] {
+    size_t f = 0, t = 0;
+    HIPC(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
   });
 }
 

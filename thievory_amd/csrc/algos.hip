@@ -160,19 +160,9 @@ static void stats_level(Graph &g, thv_stats *st, const LevelCtr &h) {
 }
 
 static ExpandArgs base_args(Graph &g) {
-  Workspace &w = g.ws;
-  ExpandArgs a;
-  memset(&a, 0, sizeof a);
-  for (int q = 0; q < 2; q++) {
-    a.qv[q] = w.qv[q].as<uint32_t>();
-    a.qe[q] = w.qe[q].as<unsigned long long>();
-    a.qc[q] = w.qc[q].as<uint32_t>();
-  }
-  a.eptr = g.store.d_eptr.as<uint64_t>();
-  a.val = w.val.as<uint32_t>();
-  a.vis = w.vis.as<uint32_t>();
-  a.C = w.C;
-  a.rec = g.rec;
+  ExpandArgs a = expand_args(g, g.ws.qs);
+  a.val = g.ws.val.as<uint32_t>();
+  a.vis = g.ws.vis.as<uint32_t>();
   return a;
 }
 
@@ -183,14 +173,14 @@ static void value_loop(Graph &g, Algo algo, thv_stats *st) {
   hipStream_t s = g.stream;
   const uint64_t words = g.n / 32 + 2;
   ExpandArgs a = base_args(g);
-  LevelCtr *d_ctr = w.ctr.as<LevelCtr>();
+  LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
   int cur = 0;
   double expand_ms = 0;
   uint32_t launches = 0;
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
-    launch_compact(g, w.front[cur].as<uint32_t>(), c, s);
+    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s);
     HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
     a.ctr = c;
     a.next = w.front[cur ^ 1].as<uint32_t>();
@@ -323,7 +313,7 @@ void run_pr(Graph &g, int mode, float alpha, float tol, int max_iter, float *out
   ExpandArgs a = base_args(g);
   a.pr_src = f1;  // push: delta, pull: contribution
   a.pr_dst = f2;  // push: residual, pull: sum
-  LevelCtr *d_ctr = w.ctr.as<LevelCtr>();
+  LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
   unsigned long long *aux = w.aux.as<unsigned long long>();
   const Algo algo = mode == THV_PR_PUSH ? A_PR_PUSH : A_PR_PULL;
   int cur = 0;
@@ -332,7 +322,7 @@ void run_pr(Graph &g, int mode, float alpha, float tol, int max_iter, float *out
   const int ug = (int)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 4096));
   for (int it = 0; it < max_iter; it++) {
     LevelCtr *c = d_ctr + (it & 1);
-    launch_compact(g, w.front[cur].as<uint32_t>(), c, s);
+    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s);
     a.ctr = c;
     HIPC(hipEventRecord(w.ev[0], s));
     launch_expand(g, algo, a, s);

@@ -1,10 +1,535 @@
-// batch.hip -- multi-source BFS / SSSP (filled in below).
-#include "internal.hpp"
+// batch.hip -- multi-source BFS / SSSP engine (multi.hpp) and thv_batch.
+//
+// No reference counterpart: the reference runs one source per process
+// (src/main.cu:152-220).  The per-level pipeline reuses the split store and
+// the partition compaction of the single-source path; only the expansion
+// kernels and the per-vertex state differ.
+#include <algorithm>
+
+#include "chunk.hpp"
+#include "multi.hpp"
 
 namespace thv {
-void run_batch(Graph &, int, const uint32_t *, uint32_t, uint64_t *, uint64_t *, uint32_t *,
-               uint32_t *, thv_stats *) {
-  fail(THV_ENOTSUP, "batched queries are not built yet");
+
+namespace {
+
+constexpr int MBS = 256;
+constexpr int ACC_HASH = 0, ACC_CNT = 64, ACC_LM = 128, RES_HASH = 129, RES_CNT = 193,
+              RES_DEPTH = 257, ACC_WORDS = 384;
+
+struct MsArgs {
+  const uint32_t *qv[2];
+  const unsigned long long *qe[2];
+  const uint32_t *qc[2];
+  const LevelCtr *ctr;
+  const uint64_t *eptr;
+  const unsigned long long *F;
+  unsigned long long *N, *seen;
+  uint32_t *nb;
+  uint32_t *dist;
+  uint32_t *full;
+  unsigned long long *acc;
+  uint64_t n;
+  uint32_t L;
+  uint32_t level;
+  uint32_t voff[64];
+};
+
+// BFS: lane per edge, lane masks staged per queue entry.
+template <int REC, int EPT>
+__global__ void __launch_bounds__(MBS) k_ms_bfs(MsArgs a) {
+  constexpr uint32_t C = MBS * EPT, NW = C / 32;
+  __shared__ uint32_t s_mask[NW], s_pre[NW];
+  __shared__ unsigned long long s_base[C + 1];
+  __shared__ unsigned long long s_m[C + 1];
+  __shared__ unsigned long long s_hash[64];
+  __shared__ unsigned int s_cnt[64];
+  __shared__ uint32_t s_val[64];
+  __shared__ unsigned long long s_lm;
+  const uint32_t tid = threadIdx.x;
+  if (tid < 64) {
+    s_hash[tid] = 0;
+    s_cnt[tid] = 0;
+    s_val[tid] = a.level + 2 - a.voff[tid];
+  }
+  if (tid == 0) s_lm = 0;
+  const ChunkSrc cs = chunk_src(a.ctr, C);
+  for (uint32_t c = blockIdx.x; c < cs.total; c += gridDim.x) {
+    int q;
+    uint32_t cc;
+    chunk_pick(cs, c, q, cc);
+    const ChunkInfo ci = chunk_info<C>(cs, a.qc, q, cc);
+    for (uint32_t t = tid; t < NW; t += MBS) s_mask[t] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < ci.nent; t += MBS) {
+      const uint32_t v = a.qv[q][ci.i0 + t];
+      const unsigned long long b = a.qe[q][ci.i0 + t];
+      const unsigned long long s = chunk_rel(b, ci.e0);
+      if (s < C) {
+        atomicOr(&s_mask[s >> 5], 1u << (s & 31));
+        s_base[t] = a.eptr[v] - b * REC;
+        s_m[t] = a.F[v];
+      }
+    }
+    __syncthreads();
+    chunk_prefix<NW>(s_mask, s_pre);
+    __syncthreads();
+    uint32_t dst[EPT], ent[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      const uint32_t l = tid + j * MBS;
+      dst[j] = 0xFFFFFFFFu;
+      ent[j] = 0;
+      if (l < ci.len) {
+        const uint32_t k = chunk_rank(s_mask, s_pre, l);
+        ent[j] = k;
+        dst[j] = *reinterpret_cast<const uint32_t *>(s_base[k] + (ci.e0 + l) * REC);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      const uint32_t w = dst[j];
+      if (w == 0xFFFFFFFFu) continue;
+      const unsigned long long m = s_m[ent[j]];
+      const unsigned long long add = m & ~a.seen[w];
+      if (!add) continue;
+      const unsigned long long gained = add & ~atomicOr(&a.seen[w], add);
+      if (!gained) continue;
+      if (atomicOr(&a.N[w], gained) == 0ull) atomicOr(&a.nb[w >> 5], 1u << (w & 31));
+      atomicOr(&s_lm, gained);
+      unsigned long long g = gained;
+      while (g) {
+        const int qq = __ffsll((long long)g) - 1;
+        g &= g - 1;
+        const uint32_t val = s_val[qq];
+        atomicAdd(&s_hash[qq], mix64(((unsigned long long)w << 32) | val));
+        atomicAdd(&s_cnt[qq], 1u);
+        if (a.full) a.full[(uint64_t)qq * a.n + w] = val;
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (tid < 64 && s_cnt[tid]) {
+    atomicAdd(&a.acc[ACC_HASH + tid], s_hash[tid]);
+    atomicAdd(&a.acc[ACC_CNT + tid], (unsigned long long)s_cnt[tid]);
+  }
+  if (tid == 0 && s_lm) atomicOr(&a.acc[ACC_LM], s_lm);
 }
-void autotune(Graph &, const char *, int *) { fail(THV_ENOTSUP, "autotuner not built yet"); }
+
+// SSSP: one wave per edge, lane q relaxes query q; two edges in flight per
+// wave for memory-level parallelism.
+template <int EPT>
+__global__ void __launch_bounds__(MBS) k_ms_sssp(MsArgs a) {
+  constexpr uint32_t C = MBS * EPT, NW = C / 32;
+  constexpr uint32_t WAVES = MBS / 64;
+  __shared__ uint32_t s_mask[NW], s_pre[NW];
+  __shared__ unsigned long long s_base[C + 1];
+  __shared__ unsigned long long s_m[C + 1];
+  __shared__ uint32_t s_v[C + 1];
+  __shared__ unsigned long long s_lm;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const uint32_t L = a.L;
+  if (tid == 0) s_lm = 0;
+  const ChunkSrc cs = chunk_src(a.ctr, C);
+  for (uint32_t c = blockIdx.x; c < cs.total; c += gridDim.x) {
+    int q;
+    uint32_t cc;
+    chunk_pick(cs, c, q, cc);
+    const ChunkInfo ci = chunk_info<C>(cs, a.qc, q, cc);
+    for (uint32_t t = tid; t < NW; t += MBS) s_mask[t] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < ci.nent; t += MBS) {
+      const uint32_t v = a.qv[q][ci.i0 + t];
+      const unsigned long long b = a.qe[q][ci.i0 + t];
+      const unsigned long long s = chunk_rel(b, ci.e0);
+      if (s < C) {
+        atomicOr(&s_mask[s >> 5], 1u << (s & 31));
+        s_base[t] = a.eptr[v] - b * 8;
+        s_m[t] = a.F[v];
+        s_v[t] = v;
+      }
+    }
+    __syncthreads();
+    chunk_prefix<NW>(s_mask, s_pre);
+    __syncthreads();
+    for (uint32_t l0 = wv; l0 < ci.len; l0 += 2 * WAVES) {
+      uint32_t w[2], wt[2], v[2];
+      unsigned long long m[2];
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const uint32_t l = l0 + j * WAVES;
+        w[j] = 0xFFFFFFFFu;
+        m[j] = 0;
+        v[j] = 0;
+        wt[j] = 0;
+        if (l < ci.len) {
+          const uint32_t k = chunk_rank(s_mask, s_pre, l);
+          const uint2 r = *reinterpret_cast<const uint2 *>(s_base[k] + (ci.e0 + l) * 8);
+          w[j] = r.x;
+          wt[j] = r.y;
+          m[j] = s_m[k];
+          v[j] = s_v[k];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        bool imp = false;
+        if (w[j] != 0xFFFFFFFFu && lane < L && ((m[j] >> lane) & 1ull)) {
+          const unsigned long long nd =
+              (unsigned long long)a.dist[(uint64_t)v[j] * L + lane] + wt[j];
+          uint32_t *dp = &a.dist[(uint64_t)w[j] * L + lane];
+          if (nd < *dp) imp = nd < atomicMin(dp, (uint32_t)nd);
+        }
+        const unsigned long long b = __ballot(imp);
+        if (b && lane == 0) {
+          if (atomicOr(&a.N[w[j]], b) == 0ull) atomicOr(&a.nb[w[j] >> 5], 1u << (w[j] & 31));
+          atomicOr(&s_lm, b);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (tid == 0 && s_lm) atomicOr(&a.acc[ACC_LM], s_lm);
+}
+
+struct AdmitArgs {
+  uint32_t count;
+  uint32_t lane[64];
+  uint32_t src[64];
+};
+
+__global__ void k_ms_admit(AdmitArgs ad, int algo, unsigned long long *F, uint32_t *fb,
+                           unsigned long long *seen, uint32_t *dist, uint32_t L,
+                           uint32_t *full, uint64_t n, unsigned long long *acc) {
+  const uint32_t i = threadIdx.x;
+  if (i >= ad.count) return;
+  const uint32_t q = ad.lane[i], s = ad.src[i];
+  const unsigned long long bit = 1ull << q;
+  if (algo == THV_ALGO_BFS) {
+    atomicOr(&seen[s], bit);
+    acc[ACC_HASH + q] = mix64(((unsigned long long)s << 32) | 1u);
+    acc[ACC_CNT + q] = 1;
+    if (full) full[(uint64_t)q * n + s] = 1;
+  } else {
+    dist[(uint64_t)s * L + q] = 1;
+  }
+  atomicOr(&F[s], bit);
+  atomicOr(&fb[s >> 5], 1u << (s & 31));
+}
+
+__global__ void k_ms_clear(unsigned long long *seen, uint64_t n, unsigned long long keep) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    seen[v] &= keep;
+}
+
+// Digest finished SSSP lanes: lane q of each wave reads column q of a row.
+__global__ void __launch_bounds__(MBS)
+k_ms_sssp_finish(uint32_t *dist, uint64_t n, uint32_t L, unsigned long long fm, uint32_t inf,
+                 bool reset, unsigned long long *acc) {
+  __shared__ unsigned long long s_h[MBS / 64][64], s_c[MBS / 64][64];
+  __shared__ uint32_t s_d[MBS / 64][64];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool mine = lane < L && ((fm >> lane) & 1ull);
+  unsigned long long h = 0, cnt = 0;
+  uint32_t dep = 0;
+  const uint64_t nw = (uint64_t)gridDim.x * (MBS / 64);
+  for (uint64_t v = blockIdx.x * (uint64_t)(MBS / 64) + wv; v < n; v += nw) {
+    if (!mine) continue;
+    uint32_t *p = &dist[v * L + lane];
+    const uint32_t d = *p;
+    if (d != inf) {
+      h += mix64((v << 32) | d);
+      cnt++;
+      dep = max(dep, d);
+      if (reset) *p = inf;
+    }
+  }
+  s_h[wv][lane] = h;
+  s_c[wv][lane] = cnt;
+  s_d[wv][lane] = dep;
+  __syncthreads();
+  if (threadIdx.x < 64 && mine) {
+    for (int i = 1; i < MBS / 64; i++) {
+      h += s_h[i][lane];
+      cnt += s_c[i][lane];
+      dep = max(dep, s_d[i][lane]);
+    }
+    if (cnt) {
+      atomicAdd(&acc[RES_HASH + lane], h);
+      atomicAdd(&acc[RES_CNT + lane], cnt);
+      atomicMax(&acc[RES_DEPTH + lane], (unsigned long long)dep);
+    }
+  }
+}
+
+__global__ void k_ms_gather(const uint32_t *dist, uint64_t n, uint32_t L, uint32_t q,
+                            uint32_t *out) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * blockDim.x)
+    out[v] = dist[v * L + q];
+}
+
+int grid_for(uint64_t n, int cap) {
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>((n + MBS - 1) / MBS, (uint64_t)cap));
+}
+
+}  // namespace
+
+MultiEngine::MultiEngine(Graph &gr, int al, uint32_t lanes, bool kf)
+    : g(gr), algo(al), L(lanes), keep_full(kf) {
+  if (algo != THV_ALGO_BFS && algo != THV_ALGO_SSSP)
+    fail(THV_EINVAL, "batched queries are BFS or SSSP");
+  if (L == 0 || L > MAXL) fail(THV_EINVAL, "lanes must be in [1, 64]");
+  if (algo == THV_ALGO_SSSP && g.rec != 8)
+    fail(THV_ENOTSUP, "SSSP needs a weighted graph (8-byte records)");
+  memset(&stats, 0, sizeof stats);
+  store_ensure(g);
+  HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  const uint64_t n = g.n, words = bitmap_words(n);
+  qs.ensure(g, 256u * (uint32_t)g.shape.ept);
+  for (int i = 0; i < 2; i++) {
+    fb[i].alloc(words * 4);
+    F[i].alloc((n + 1) * 8);
+    HIPC(hipMemsetAsync(fb[i].p, 0, words * 4, stream));
+    HIPC(hipMemsetAsync(F[i].p, 0, (n + 1) * 8, stream));
+  }
+  if (algo == THV_ALGO_BFS) {
+    seen.alloc((n + 1) * 8);
+    HIPC(hipMemsetAsync(seen.p, 0, (n + 1) * 8, stream));
+    if (keep_full) full.alloc((uint64_t)L * n * 4 + 4);
+  } else {
+    dist.alloc((uint64_t)L * n * 4 + 4);
+    if (n) launch_fill_u32(dist.as<uint32_t>(), (uint32_t)(n + 1), (uint64_t)L * n, stream);
+    if (keep_full) full.alloc(n * 4 + 4);
+  }
+  acc.alloc(ACC_WORDS * 8);
+  HIPC(hipMemsetAsync(acc.p, 0, ACC_WORDS * 8, stream));
+  HIPC(hipHostMalloc((void **)&h_acc, ACC_WORDS * 8, hipHostMallocDefault));
+  HIPC(hipHostMalloc((void **)&h_ctr, 2 * sizeof(LevelCtr), hipHostMallocDefault));
+  HIPC(hipStreamSynchronize(stream));
+}
+
+MultiEngine::~MultiEngine() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+  if (h_acc) (void)hipHostFree(h_acc);
+  if (h_ctr) (void)hipHostFree(h_ctr);
+}
+
+uint64_t MultiEngine::bytes_needed(const Graph &g, int algo, uint32_t lanes, bool keep_full) {
+  const uint64_t n = g.n, words = bitmap_words(n);
+  uint64_t b = 2 * (words * 4 + (n + 1) * 8);                       // fb, F
+  b += 2 * ((n + 1) * 12) + 2 * (g.m / 1024 + 2) * 4;                // queues
+  if (algo == THV_ALGO_BFS) b += (n + 1) * 8 + (keep_full ? (uint64_t)lanes * n * 4 : 0);
+  else b += (uint64_t)lanes * n * 4 + (keep_full ? n * 4 : 0);
+  return b;
+}
+
+uint64_t MultiEngine::free_mask() const {
+  const uint64_t all = L == 64 ? ~0ull : ((1ull << L) - 1);
+  return all & ~active;
+}
+
+int MultiEngine::admit(uint32_t src, uint64_t tag) {
+  if (src >= g.n) fail(THV_EINVAL, "source vertex out of range");
+  const uint64_t fm = free_mask();
+  if (!fm) return -1;
+  const int q = __builtin_ctzll(fm);
+  active |= 1ull << q;
+  src_of[q] = src;
+  tag_of[q] = tag;
+  admit_q.emplace_back((uint32_t)q, src);
+  return q;
+}
+
+void MultiEngine::launch_level() {
+  const uint64_t n = g.n;
+  const uint64_t words = bitmap_words(n);
+  launched_expand = false;
+  launched_finish = false;
+  if (pend_clear) {
+    hipLaunchKernelGGL(k_ms_clear, dim3(grid_for(n, 8192)), dim3(MBS), 0, stream,
+                       seen.as<unsigned long long>(), n, ~pend_clear);
+    pend_clear = 0;
+  }
+  if (pend_finish) {
+    HIPC(hipMemsetAsync(acc.as<unsigned long long>() + RES_HASH, 0, 192 * 8, stream));
+    hipLaunchKernelGGL(k_ms_sssp_finish, dim3(grid_for(n, 2048)), dim3(MBS), 0, stream,
+                       dist.as<uint32_t>(), n, L, pend_finish, (uint32_t)(n + 1), true,
+                       acc.as<unsigned long long>());
+    finalizing = pend_finish;
+    pend_finish = 0;
+    launched_finish = true;
+  }
+  if (!admit_q.empty()) {
+    AdmitArgs ad;
+    memset(&ad, 0, sizeof ad);
+    for (auto &x : admit_q) {
+      const uint32_t q = x.first;
+      if (algo == THV_ALGO_BFS && keep_full)
+        launch_fill_u32(full.as<uint32_t>() + (uint64_t)q * n, (uint32_t)(n + 1), n, stream);
+      ad.lane[ad.count] = q;
+      ad.src[ad.count] = x.second;
+      ad.count++;
+      voff[q] = level;
+      last_val[q] = 1;
+    }
+    hipLaunchKernelGGL(k_ms_admit, dim3(1), dim3(64), 0, stream, ad, algo,
+                       F[cur].as<unsigned long long>(), fb[cur].as<uint32_t>(),
+                       seen.as<unsigned long long>(), dist.as<uint32_t>(), L,
+                       full.as<uint32_t>(), n, acc.as<unsigned long long>());
+    admit_q.clear();
+  }
+  if (active) {
+    HIPC(hipMemsetAsync(F[cur ^ 1].p, 0, (n + 1) * 8, stream));
+    HIPC(hipMemsetAsync(fb[cur ^ 1].p, 0, words * 4, stream));
+    HIPC(hipMemsetAsync(acc.as<unsigned long long>() + ACC_LM, 0, 8, stream));
+    LevelCtr *c = qs.ctr.as<LevelCtr>() + (level & 1);
+    launch_compact(g, qs, fb[cur].as<uint32_t>(), c, stream);
+    MsArgs a;
+    memset(&a, 0, sizeof a);
+    for (int i = 0; i < 2; i++) {
+      a.qv[i] = qs.qv[i].as<uint32_t>();
+      a.qe[i] = qs.qe[i].as<unsigned long long>();
+      a.qc[i] = qs.qc[i].as<uint32_t>();
+    }
+    a.ctr = c;
+    a.eptr = g.store.d_eptr.as<uint64_t>();
+    a.F = F[cur].as<unsigned long long>();
+    a.N = F[cur ^ 1].as<unsigned long long>();
+    a.seen = seen.as<unsigned long long>();
+    a.nb = fb[cur ^ 1].as<uint32_t>();
+    a.dist = dist.as<uint32_t>();
+    a.full = algo == THV_ALGO_BFS && keep_full ? full.as<uint32_t>() : nullptr;
+    a.acc = acc.as<unsigned long long>();
+    a.n = n;
+    a.L = L;
+    a.level = level;
+    for (uint32_t q = 0; q < MAXL; q++) a.voff[q] = voff[q];
+    const int grid = g.shape.blocks_per_cu * g.cus;
+    const bool r8 = g.rec == 8;
+    if (qs.C != 256u * (uint32_t)g.shape.ept) fail(THV_ESTATE, "queue chunk size mismatch");
+    if (algo == THV_ALGO_BFS) {
+      if (g.shape.ept == 4) {
+        if (r8) hipLaunchKernelGGL((k_ms_bfs<8, 4>), dim3(grid), dim3(MBS), 0, stream, a);
+        else hipLaunchKernelGGL((k_ms_bfs<4, 4>), dim3(grid), dim3(MBS), 0, stream, a);
+      } else {
+        if (r8) hipLaunchKernelGGL((k_ms_bfs<8, 8>), dim3(grid), dim3(MBS), 0, stream, a);
+        else hipLaunchKernelGGL((k_ms_bfs<4, 8>), dim3(grid), dim3(MBS), 0, stream, a);
+      }
+    } else {
+      if (g.shape.ept == 4) hipLaunchKernelGGL((k_ms_sssp<4>), dim3(grid), dim3(MBS), 0, stream, a);
+      else hipLaunchKernelGGL((k_ms_sssp<8>), dim3(grid), dim3(MBS), 0, stream, a);
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(&h_ctr[level & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, stream));
+    launched_expand = true;
+  }
+  HIPC(hipMemcpyAsync(h_acc, acc.p, ACC_WORDS * 8, hipMemcpyDeviceToHost, stream));
+}
+
+void MultiEngine::finish_level(std::vector<LaneResult> &done) {
+  HIPC(hipStreamSynchronize(stream));
+  if (launched_finish) {
+    for (uint32_t q = 0; q < L; q++) {
+      if (!((finalizing >> q) & 1ull)) continue;
+      // the lane may already hold a new query; its finished one was
+      // recorded in fin_tag/fin_src when the digest was launched
+      LaneResult r;
+      r.tag = fin_tag[q];
+      r.src = fin_src[q];
+      r.reached = h_acc[RES_CNT + q];
+      r.hash = h_acc[RES_HASH + q];
+      r.depth = (uint32_t)h_acc[RES_DEPTH + q];
+      done.push_back(r);
+    }
+    finalizing = 0;
+  }
+  just_finished = 0;
+  if (!launched_expand) return;
+  const LevelCtr &h = h_ctr[level & 1];
+  stats.edges_traversed += h.qw[0] + h.qw[1];
+  stats.host_bytes += h.qw[0] * (uint64_t)g.rec;
+  stats.hbm_bytes += h.qw[1] * (uint64_t)g.rec;
+  stats.iterations++;
+  stats.expand_launches++;
+  const uint64_t lm = h_acc[ACC_LM];
+  for (uint32_t q = 0; q < L; q++) {
+    const uint64_t bit = 1ull << q;
+    if (!(active & bit)) continue;
+    if (lm & bit) {
+      last_val[q] = level + 2 - voff[q];
+      continue;
+    }
+    active &= ~bit;
+    just_finished |= bit;
+    if (algo == THV_ALGO_BFS) {
+      LaneResult r;
+      r.tag = tag_of[q];
+      r.src = src_of[q];
+      r.reached = h_acc[ACC_CNT + q];
+      r.hash = h_acc[ACC_HASH + q];
+      r.depth = last_val[q];
+      done.push_back(r);
+      pend_clear |= bit;
+    } else {
+      pend_finish |= bit;
+      fin_tag[q] = tag_of[q];
+      fin_src[q] = src_of[q];
+    }
+  }
+  cur ^= 1;
+  level++;
+}
+
+void MultiEngine::copy_full(uint32_t q, uint32_t *out) {
+  if (!keep_full) fail(THV_ESTATE, "engine does not keep value arrays");
+  const uint64_t n = g.n;
+  if (algo == THV_ALGO_BFS) {
+    HIPC(hipMemcpyAsync(out, full.as<uint32_t>() + (uint64_t)q * n, n * 4,
+                        hipMemcpyDeviceToHost, stream));
+  } else {
+    hipLaunchKernelGGL(k_ms_gather, dim3(grid_for(n, 8192)), dim3(MBS), 0, stream,
+                       dist.as<uint32_t>(), n, L, q, full.as<uint32_t>());
+    HIPC(hipMemcpyAsync(out, full.p, n * 4, hipMemcpyDeviceToHost, stream));
+  }
+  HIPC(hipStreamSynchronize(stream));
+}
+
+void run_batch(Graph &g, int algo, const uint32_t *srcs, uint32_t k, uint64_t *hash,
+               uint64_t *reached, uint32_t *depth, uint32_t *full, thv_stats *st) {
+  if (k == 0) {
+    if (st) memset(st, 0, sizeof *st);
+    return;
+  }
+  for (uint32_t i = 0; i < k; i++)
+    if (srcs[i] >= g.n) fail(THV_EINVAL, "source vertex out of range");
+  MultiEngine e(g, algo, k, full != nullptr);
+  for (uint32_t i = 0; i < k; i++) e.admit(srcs[i], i);
+  std::vector<LaneResult> done;
+  const uint64_t t0 = now_ns();
+  while (e.busy()) {
+    e.launch_level();
+    e.finish_level(done);
+    if (full && e.just_finished) {
+      for (uint32_t q = 0; q < k; q++)
+        if ((e.just_finished >> q) & 1ull) e.copy_full(q, full + (uint64_t)e.tag_of[q] * g.n);
+    }
+  }
+  if (st) {
+    *st = e.stats;
+    st->ms = (now_ns() - t0) * 1e-6;
+  }
+  for (auto &r : done) {
+    if (hash) hash[r.tag] = r.hash;
+    if (reached) reached[r.tag] = r.reached;
+    if (depth) depth[r.tag] = r.depth;
+  }
+}
+
 }  // namespace thv

@@ -5,8 +5,8 @@
 // 196-287; CC32_* src/cc/cc_kernels.cu; SSSP32_* src/sssp/sssp_kernels.cu;
 // PR32_* src/pr/pr_kernels.cu) with one kernel per algorithm that walks the
 // level's work space -- the concatenated edge lists of the queued vertices --
-// in chunks of C = 256 * EPT edges.  Host-resident chunks come first so
-// their zero-copy reads are in flight while HBM chunks are processed.
+// in chunks of C = 256 * EPT edges (chunk.hpp), host-resident and HBM
+// chunks interleaved.
 //
 // Per chunk (one 256-thread block):
 //   1. the chunk-start table names the first queue entry; entries whose
@@ -19,7 +19,8 @@
 //      issued before any is consumed, then each edge is relaxed.
 // Hubs are therefore split across many blocks and low-degree vertices share
 // a wave, with consecutive lanes reading consecutive addresses whenever the
-// queued vertices are adjacent (coalesced 256-byte wave reads).
+// queued vertices are adjacent (coalesced 256-byte wave reads, over the
+// host link as well as from HBM).
 //
 // Relaxation rules (identical fixpoints to the reference):
 //   BFS  : first visit wins (atomicOr on the visited bitmap), value = level
@@ -28,8 +29,7 @@
 //   SSSP : atomicMin(val[w], val[v] + weight) (sssp_kernels.cu:124-141)
 //   PR   : push atomicAdd(res[w], delta[v]) (pr_kernels.cu:539-542);
 //          pull sum[v] += contrib[u] over in-edges (pr_kernels.cu:45-51)
-#include "internal.hpp"
-#include "scan.hpp"
+#include "chunk.hpp"
 
 namespace thv {
 
@@ -45,31 +45,23 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   __shared__ uint32_t s_u[C + 1];  // CC/SSSP: source value; PR pull: vertex id
   __shared__ float s_f[C + 1];     // PR push: delta; PR pull: partial sum
 
-  const unsigned long long Wh = a.ctr->qw[Q_HOST], Wd = a.ctr->qw[Q_DEV];
-  const unsigned long long nqh = a.ctr->qn[Q_HOST], nqd = a.ctr->qn[Q_DEV];
-  const uint32_t Hc = (uint32_t)((Wh + C - 1) / C), Dc = (uint32_t)((Wd + C - 1) / C);
+  const ChunkSrc cs = chunk_src(a.ctr, C);
   const uint32_t tid = threadIdx.x;
 
-  for (uint32_t c = blockIdx.x; c < Hc + Dc; c += gridDim.x) {
-    const int q = c < Hc ? Q_HOST : Q_DEV;
-    const uint32_t cc = q == Q_HOST ? c : c - Hc;
-    const uint32_t nch = q == Q_HOST ? Hc : Dc;
-    const unsigned long long W = q == Q_HOST ? Wh : Wd;
-    const unsigned long long nq = q == Q_HOST ? nqh : nqd;
+  for (uint32_t c = blockIdx.x; c < cs.total; c += gridDim.x) {
+    int q;
+    uint32_t cc;
+    chunk_pick(cs, c, q, cc);
+    const ChunkInfo ci = chunk_info<C>(cs, a.qc, q, cc);
     const uint32_t *__restrict__ qv = a.qv[q];
     const unsigned long long *__restrict__ qe = a.qe[q];
-    const unsigned long long e0 = (unsigned long long)cc * C;
-    const uint32_t len = (uint32_t)min((unsigned long long)C, W - e0);
-    const uint32_t i0 = a.qc[q][cc];
-    const uint32_t i1 = cc + 1 < nch ? a.qc[q][cc + 1] : (uint32_t)(nq - 1);
-    const uint32_t nent = i1 - i0 + 1;
 
     for (uint32_t t = tid; t < NW; t += EBS) s_mask[t] = 0;
     __syncthreads();
-    for (uint32_t t = tid; t < nent; t += EBS) {
-      const uint32_t v = qv[i0 + t];
-      const unsigned long long b = qe[i0 + t];
-      const unsigned long long s = b > e0 ? b - e0 : 0;
+    for (uint32_t t = tid; t < ci.nent; t += EBS) {
+      const uint32_t v = qv[ci.i0 + t];
+      const unsigned long long b = qe[ci.i0 + t];
+      const unsigned long long s = chunk_rel(b, ci.e0);
       if (s < C) {
         atomicOr(&s_mask[s >> 5], 1u << (s & 31));
         s_base[t] = a.eptr[v] - b * REC;
@@ -82,11 +74,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       }
     }
     __syncthreads();
-    if (tid < 64) {
-      uint32_t pc = tid < NW ? __popc(s_mask[tid]) : 0u;
-      uint32_t inc = wave_incl_scan(pc);
-      if (tid < NW) s_pre[tid] = inc - pc;
-    }
+    chunk_prefix<NW>(s_mask, s_pre);
     __syncthreads();
 
     uint32_t dst[EPT], wt[EPT], ent[EPT];
@@ -96,11 +84,10 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       dst[j] = 0xFFFFFFFFu;
       wt[j] = 0;
       ent[j] = 0;
-      if (l < len) {
-        const uint32_t wd = l >> 5;
-        const uint32_t k = s_pre[wd] + __popc(s_mask[wd] & (0xFFFFFFFFu >> (31 - (l & 31)))) - 1;
+      if (l < ci.len) {
+        const uint32_t k = chunk_rank(s_mask, s_pre, l);
         ent[j] = k;
-        const unsigned long long addr = s_base[k] + (e0 + l) * REC;
+        const unsigned long long addr = s_base[k] + (ci.e0 + l) * REC;
         if (REC == 8) {
           const uint2 r = *reinterpret_cast<const uint2 *>(addr);
           dst[j] = r.x;
@@ -140,9 +127,8 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
     }
     if (ALGO == A_PR_PULL) {
       __syncthreads();
-      for (uint32_t t = tid; t < nent; t += EBS) {
-        const unsigned long long b = qe[i0 + t];
-        if ((b > e0 ? b - e0 : 0) < C) unsafeAtomicAdd(&a.pr_dst[s_u[t]], s_f[t]);
+      for (uint32_t t = tid; t < ci.nent; t += EBS) {
+        if (chunk_rel(qe[ci.i0 + t], ci.e0) < C) unsafeAtomicAdd(&a.pr_dst[s_u[t]], s_f[t]);
       }
     }
     __syncthreads();
@@ -165,6 +151,8 @@ static void launch_t(Graph &g, const ExpandArgs &args, hipStream_t s) {
 }
 
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
+  if (args.C != 256u * (uint32_t)g.shape.ept)
+    fail(THV_ESTATE, "queue chunk size does not match the launch shape");
   const bool r8 = g.rec == 8;
   switch (a) {
     case A_BFS: r8 ? launch_t<A_BFS, 8>(g, args, s) : launch_t<A_BFS, 4>(g, args, s); break;

@@ -134,8 +134,8 @@ k_write(const uint32_t *__restrict__ front, const uint32_t *__restrict__ pv, uin
   }
 }
 
-void launch_compact(Graph &g, const uint32_t *front, LevelCtr *ctr, hipStream_t s) {
-  Workspace &w = g.ws;
+void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
+                    hipStream_t s) {
   Store &st = g.store;
   const uint32_t P = st.P;
   if (P == 0) return;
@@ -153,6 +153,20 @@ void launch_compact(Graph &g, const uint32_t *front, LevelCtr *ctr, hipStream_t 
                      w.qc[0].as<uint32_t>(), w.qv[1].as<uint32_t>(),
                      w.qe[1].as<unsigned long long>(), w.qc[1].as<uint32_t>(), w.C);
   HIPC(hipGetLastError());
+}
+
+ExpandArgs expand_args(const Graph &g, const QueueSet &qs) {
+  ExpandArgs a;
+  memset(&a, 0, sizeof a);
+  for (int q = 0; q < 2; q++) {
+    a.qv[q] = qs.qv[q].as<uint32_t>();
+    a.qe[q] = qs.qe[q].as<unsigned long long>();
+    a.qc[q] = qs.qc[q].as<uint32_t>();
+  }
+  a.eptr = g.store.d_eptr.as<uint64_t>();
+  a.C = qs.C;
+  a.rec = g.rec;
+  return a;
 }
 
 }  // namespace thv

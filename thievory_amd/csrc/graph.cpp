@@ -116,7 +116,20 @@ void graph_build_tables(Graph &g) {
   }
   g.d_deg.alloc(g.n * 4);
   if (g.n) HIPC(hipMemcpy(g.d_deg.p, deg.data(), g.n * 4, hipMemcpyHostToDevice));
-  store_configure(g, UINT64_MAX, 1u << 16);
+}
+
+// Default store, built on first use when the caller configured none: like
+// CSR::SetNumStaticEdges (graph.cuh:306-341) the edge budget is 90% of the
+// HBM still free after the per-vertex tables and a workspace reserve, but
+// filled by partition density rather than as a vertex prefix.
+void store_ensure(Graph &g) {
+  if (g.store.configured) return;
+  size_t fr = 0, tot = 0;
+  HIPC(hipMemGetInfo(&fr, &tot));
+  const uint64_t reserve = g.n * 96 + (g.m / 256 + 1) * 16 + (256ull << 20);
+  const uint64_t avail = fr > reserve ? (uint64_t)((fr - reserve) * 0.9) : 0;
+  const uint64_t all = g.m * (uint64_t)g.rec;
+  store_configure(g, avail >= all ? UINT64_MAX : avail, 1u << 16);
 }
 
 // Greedy contiguous vertex ranges of at most part_edges edges; a vertex
@@ -143,6 +156,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   if (g.n >= 0xFFFFFFFFull) fail(THV_ENOTSUP, "graphs need fewer than 2^32-1 vertices");
   Store &s = g.store;
   HIPC(hipStreamSynchronize(g.stream));
+  s.configured = false;
   s.part_edges = part_edges;
   s.budget = budget;
   make_partitions(g, part_edges, s.pv);
@@ -224,29 +238,38 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   HIPC(hipMemcpy(s.d_pv.p, s.pv.data(), (s.P + 1) * 4, hipMemcpyHostToDevice));
   s.d_pres.alloc(s.P ? s.P : 1);
   if (s.P) HIPC(hipMemcpy(s.d_pres.p, s.pres.data(), s.P, hipMemcpyHostToDevice));
+  s.configured = true;
+}
+
+void QueueSet::ensure(const Graph &g, uint32_t chunk) {
+  if (P != g.store.P || !pcnt.p) {
+    const uint64_t np = g.store.P + 1;
+    pcnt.alloc(np * 4);
+    pdeg.alloc(np * 8);
+    pbc.alloc(np * 4);
+    pbe.alloc(np * 8);
+    P = g.store.P;
+  }
+  if (n == g.n && m == g.m && C == chunk && ctr.p) return;
+  for (int q = 0; q < 2; q++) {
+    qv[q].alloc((g.n + 1) * 4);
+    qe[q].alloc((g.n + 1) * 8);
+    qc[q].alloc((g.m / chunk + 2) * 4);
+  }
+  ctr.alloc(2 * sizeof(LevelCtr));
+  n = g.n;
+  m = g.m;
+  C = chunk;
 }
 
 void workspace_ensure(Graph &g) {
+  store_ensure(g);
   Workspace &w = g.ws;
-  const uint32_t C = 256u * (uint32_t)g.shape.ept;
-  if (w.P != g.store.P || !w.pcnt.p) {
-    const uint64_t P = g.store.P + 1;
-    w.pcnt.alloc(P * 4);
-    w.pdeg.alloc(P * 8);
-    w.pbc.alloc(P * 4);
-    w.pbe.alloc(P * 8);
-    w.P = g.store.P;
-  }
-  if (w.n == g.n && w.m == g.m && w.C == C && w.val.p) return;
-  const uint64_t words = g.n / 32 + 2;
+  w.qs.ensure(g, 256u * (uint32_t)g.shape.ept);
+  if (w.n == g.n && w.m == g.m && w.val.p) return;
+  const uint64_t words = bitmap_words(g.n);
   for (auto &f : w.front) f.alloc(words * 4);
   w.vis.alloc(words * 4);
-  for (int q = 0; q < 2; q++) {
-    w.qv[q].alloc((g.n + 1) * 4);
-    w.qe[q].alloc((g.n + 1) * 8);
-    w.qc[q].alloc((g.m / C + 2) * 4);
-  }
-  w.ctr.alloc(2 * sizeof(LevelCtr));
   w.aux.alloc(8 * sizeof(unsigned long long));
   w.val.alloc((g.n + 1) * 4);
   if (!w.h_ctr) HIPC(hipHostMalloc((void **)&w.h_ctr, 2 * sizeof(LevelCtr), hipHostMallocDefault));
@@ -256,7 +279,6 @@ void workspace_ensure(Graph &g) {
     if (!e) HIPC(hipEventCreate(&e));
   w.n = g.n;
   w.m = g.m;
-  w.C = C;
 }
 
 }  // namespace thv

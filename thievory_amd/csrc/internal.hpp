@@ -70,6 +70,7 @@ struct HostPinned {  // hipHostMalloc'd, mapped into the GPU address space
 // ---- store -------------------------------------------------------------------
 
 struct Store {
+  bool configured = false;
   uint32_t part_edges = 1u << 16;
   uint64_t budget = UINT64_MAX;      // device edge budget in bytes
   uint32_t P = 0;                    // partitions
@@ -93,15 +94,26 @@ struct LevelCtr {
   unsigned int pad[2];
 };
 
-struct Workspace {
+struct Graph;
+
+// Frontier queues of one traversal engine: filled by the compaction kernels,
+// consumed by an expansion kernel.  Engines that run concurrently on
+// different streams own separate QueueSets.
+struct QueueSet {
   uint64_t n = 0, m = 0;
   uint32_t C = 0;             // edges per expansion chunk
-  DevMem front[2];            // frontier bitmaps (n bits, word padded)
-  DevMem vis;                 // BFS visited bitmap
+  uint32_t P = 0;             // partitions the per-partition arrays cover
   DevMem qv[2], qe[2], qc[2]; // queue vertex ids, edge prefixes, chunk starts
   DevMem ctr;                 // LevelCtr[2]
   DevMem pcnt, pdeg, pbc, pbe; // per partition: active count / edges, bases
-  uint32_t P = 0;
+  void ensure(const Graph &g, uint32_t C);
+};
+
+struct Workspace {
+  uint64_t n = 0, m = 0;
+  QueueSet qs;
+  DevMem front[2];            // frontier bitmaps (n bits, word padded)
+  DevMem vis;                 // BFS visited bitmap
   DevMem val;                 // u32 values (BFS / CC / SSSP)
   DevMem f0, f1, f2;          // PR arrays (fp32)
   DevMem aux;                 // misc device counters (8 x u64)
@@ -136,6 +148,7 @@ struct Graph {
   int device = 0;
   int cus = 256;
   std::mutex mu;              // serialises algorithm calls on one handle
+  std::atomic<int> servers{0};  // running servers (store must stay fixed)
   ~Graph();
 };
 
@@ -143,6 +156,7 @@ struct Graph {
 void graph_init_device(Graph &g);
 void graph_build_tables(Graph &g);
 void store_configure(Graph &g, uint64_t budget, uint32_t part_edges);
+void store_ensure(Graph &g);
 void workspace_ensure(Graph &g);
 uint64_t now_ns();
 
@@ -165,8 +179,10 @@ struct ExpandArgs {
   int rec;
 };
 
-void launch_compact(Graph &g, const uint32_t *front, LevelCtr *ctr,
+void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s);
+ExpandArgs expand_args(const Graph &g, const QueueSet &qs);
+inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);
 
 // algos.hip

@@ -1,10 +1,251 @@
-// serve.cpp -- continuous-batching query server (filled in below).
-#include "internal.hpp"
+// serve.cpp -- continuous-batching query server (no reference counterpart).
+//
+// One server thread owns two multi-source engines (multi.hpp), BFS and
+// SSSP, each on its own HIP stream so their levels overlap on the device.
+// Between levels it admits waiting queries into free lanes (a lane is the
+// unit of frontier/scratch memory, so admission is bounded by the memory the
+// engines were sized for), launches one level of each busy engine, then
+// harvests the lanes that finished.  Submit and poll are thread-safe; the
+// graph stays resident for the server's lifetime (store reconfiguration is
+// refused meanwhile).
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "multi.hpp"
+
+namespace thv {
+void set_last_error(const std::string &m);
+
+struct Server {
+  Graph *g = nullptr;
+  thv_serve_cfg cfg{};
+  std::unique_ptr<MultiEngine> eng[2];  // [0] BFS, [1] SSSP
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  struct Req {
+    uint64_t ticket;
+    uint32_t src;
+  };
+  std::deque<Req> waiting[2];
+  std::unordered_map<uint64_t, thv_result> res;
+  std::unordered_map<uint64_t, std::vector<uint32_t>> full;
+  uint64_t next_ticket = 1;
+  uint64_t outstanding = 0;
+  bool stop = false;
+  int err = 0;
+  std::string errmsg;
+  uint64_t t0 = 0;
+  static constexpr size_t MAX_WAITING = 1u << 20;
+
+  double now_ms() const { return (now_ns() - t0) * 1e-6; }
+  void loop();
+  void fail_all(int code, const std::string &msg);
+};
+
+void Server::fail_all(int code, const std::string &msg) {
+  std::lock_guard<std::mutex> lk(mu);
+  err = code;
+  errmsg = msg;
+  for (auto &kv : res)
+    if (kv.second.status == 1) kv.second.status = code;
+  for (auto &w : waiting) w.clear();
+  outstanding = 0;
+  cv_done.notify_all();
+}
+
+void Server::loop() {
+  std::vector<LaneResult> done;
+  try {
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        auto idle = [&] {
+          for (int a = 0; a < 2; a++)
+            if ((eng[a] && eng[a]->busy()) || !waiting[a].empty()) return false;
+          return true;
+        };
+        cv_work.wait(lk, [&] { return stop || !idle(); });
+        if (stop && idle()) break;
+        for (int a = 0; a < 2; a++) {
+          if (!eng[a]) continue;
+          while (!waiting[a].empty() && eng[a]->free_lanes()) {
+            const Req r = waiting[a].front();
+            waiting[a].pop_front();
+            eng[a]->admit(r.src, r.ticket);
+            res[r.ticket].start_ms = now_ms();
+          }
+        }
+      }
+      bool launched[2] = {false, false};
+      for (int a = 0; a < 2; a++)
+        if (eng[a] && eng[a]->busy()) {
+          eng[a]->launch_level();
+          launched[a] = true;
+        }
+      done.clear();
+      for (int a = 0; a < 2; a++) {
+        if (!launched[a]) continue;
+        eng[a]->finish_level(done);
+        if (cfg.keep_full && eng[a]->just_finished) {
+          for (uint32_t q = 0; q < eng[a]->L; q++) {
+            if (!((eng[a]->just_finished >> q) & 1ull)) continue;
+            std::vector<uint32_t> v(g->n);
+            eng[a]->copy_full(q, v.data());
+            std::lock_guard<std::mutex> lk(mu);
+            full[eng[a]->tag_of[q]] = std::move(v);
+          }
+        }
+      }
+      if (!done.empty()) {
+        std::lock_guard<std::mutex> lk(mu);
+        const double t = now_ms();
+        for (const LaneResult &r : done) {
+          thv_result &x = res[r.tag];
+          x.status = 0;
+          x.depth = r.depth;
+          x.reached = r.reached;
+          x.hash = r.hash;
+          x.done_ms = t;
+          if (outstanding) outstanding--;
+        }
+        cv_done.notify_all();
+      }
+    }
+  } catch (const Error &e) {
+    fail_all(e.code, e.what());
+  } catch (const std::exception &e) {
+    fail_all(THV_EHIP, e.what());
+  }
+}
+
+}  // namespace thv
+
+using namespace thv;
+
+template <class F>
+static int guard(F f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return THV_ENOMEM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return THV_EHIP;
+  }
+}
 
 extern "C" {
-int thv_serve_start(thv_graph *, const thv_serve_cfg *, thv_server **) { return THV_ENOTSUP; }
-int thv_serve_submit(thv_server *, int, uint32_t, uint64_t *) { return THV_ENOTSUP; }
-int thv_serve_poll(thv_server *, uint64_t, thv_result *, uint32_t *) { return THV_ENOTSUP; }
-int thv_serve_drain(thv_server *) { return THV_ENOTSUP; }
-void thv_serve_stop(thv_server *) {}
+
+int thv_serve_start(thv_graph *gh, const thv_serve_cfg *cfg, thv_server **out) {
+  return guard([&] {
+    if (!gh || !cfg || !out) fail(THV_EINVAL, "null argument");
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::unique_ptr<Server> s(new Server());
+    s->g = g;
+    s->cfg = *cfg;
+    uint32_t lanes = cfg->max_lanes ? std::min<uint32_t>(cfg->max_lanes, 64) : 64;
+    uint64_t budget = cfg->scratch_bytes;
+    if (budget == 0) {
+      size_t fr = 0, tot = 0;
+      HIPC(hipMemGetInfo(&fr, &tot));
+      budget = fr / 2;
+    }
+    const bool kf = cfg->keep_full != 0;
+    // BFS lanes first (cheap: 3 masks per vertex), SSSP lanes get the rest,
+    // halving until the lane-packed distances fit the scratch budget.
+    const uint64_t need_bfs = MultiEngine::bytes_needed(*g, THV_ALGO_BFS, lanes, kf);
+    if (need_bfs > budget)
+      fail(THV_ENOMEM, "scratch budget too small for the BFS engine");
+    s->eng[0].reset(new MultiEngine(*g, THV_ALGO_BFS, lanes, kf));
+    if (g->rec == 8) {
+      uint32_t sl = lanes;
+      while (sl && MultiEngine::bytes_needed(*g, THV_ALGO_SSSP, sl, kf) > budget - need_bfs) sl /= 2;
+      if (sl) s->eng[1].reset(new MultiEngine(*g, THV_ALGO_SSSP, sl, kf));
+    }
+    g->servers++;
+    s->t0 = now_ns();
+    Server *sp = s.get();
+    s->th = std::thread([sp] { sp->loop(); });
+    *out = reinterpret_cast<thv_server *>(s.release());
+    return THV_OK;
+  });
 }
+
+int thv_serve_submit(thv_server *sh, int algo, uint32_t src, uint64_t *ticket) {
+  return guard([&] {
+    Server *s = reinterpret_cast<Server *>(sh);
+    if (!s || !ticket) fail(THV_EINVAL, "null argument");
+    const int a = algo == THV_ALGO_BFS ? 0 : algo == THV_ALGO_SSSP ? 1 : -1;
+    if (a < 0) fail(THV_EINVAL, "serving takes BFS or SSSP queries");
+    if (!s->eng[a]) fail(THV_ENOTSUP, "no engine for this algorithm (unweighted graph or memory)");
+    if (src >= s->g->n) fail(THV_EINVAL, "source vertex out of range");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->err) fail(THV_ESTATE, "server failed: " + s->errmsg);
+    if (s->stop) fail(THV_ESTATE, "server is stopping");
+    if (s->waiting[a].size() >= Server::MAX_WAITING) fail(THV_EBUSY, "admission queue full");
+    const uint64_t t = s->next_ticket++;
+    thv_result r;
+    memset(&r, 0, sizeof r);
+    r.ticket = t;
+    r.algo = algo;
+    r.status = 1;
+    r.src = src;
+    r.submit_ms = s->now_ms();
+    s->res[t] = r;
+    s->waiting[a].push_back({t, src});
+    s->outstanding++;
+    *ticket = t;
+    s->cv_work.notify_one();
+    return THV_OK;
+  });
+}
+
+int thv_serve_poll(thv_server *sh, uint64_t ticket, thv_result *out, uint32_t *fullv) {
+  return guard([&] {
+    Server *s = reinterpret_cast<Server *>(sh);
+    if (!s || !out) fail(THV_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->res.find(ticket);
+    if (it == s->res.end()) fail(THV_EINVAL, "unknown ticket");
+    *out = it->second;
+    if (fullv && it->second.status == 0) {
+      auto f = s->full.find(ticket);
+      if (f == s->full.end()) fail(THV_ESTATE, "server does not keep value arrays");
+      memcpy(fullv, f->second.data(), f->second.size() * 4);
+    }
+    return THV_OK;
+  });
+}
+
+int thv_serve_drain(thv_server *sh) {
+  return guard([&] {
+    Server *s = reinterpret_cast<Server *>(sh);
+    if (!s) fail(THV_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+    if (s->err) fail(s->err, "server failed: " + s->errmsg);
+    return THV_OK;
+  });
+}
+
+void thv_serve_stop(thv_server *sh) {
+  Server *s = reinterpret_cast<Server *>(sh);
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  s->g->servers--;
+  delete s;
+}
+
+}  // extern "C"
