@@ -57,6 +57,180 @@ def cpu_baseline(g, src, seconds, label):
                       f"(oracle/oracle.c orc_bfs_mt, OpenMP top-down), {total:.1f} s"}, lv
 
 
+def pct(xs, p):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))] if xs else None
+
+
+def serve_stream(T, g, queries, rate, seed):
+    """Submits (algo, src) queries to a fresh server, at Poisson arrivals of
+    `rate` per second (None: all at once); returns throughput and latency."""
+    srv = T.Server(g, max_lanes=64)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(queries)) if rate else np.zeros(len(queries))
+    t0 = time.perf_counter()
+    due = 0.0
+    tickets = []
+    for (a, s), gap in zip(queries, gaps):
+        due += gap
+        while rate:
+            left = due - (time.perf_counter() - t0)
+            if left <= 0:
+                break
+            time.sleep(min(0.0005, left))
+        tickets.append((a, srv.submit(a, s)))
+    srv.drain()
+    wall = time.perf_counter() - t0
+    res = [(a, srv.poll(t)) for a, t in tickets]
+    srv.stop()
+    assert all(r["status"] == 0 for _, r in res)
+    first = min(r["submit_ms"] for _, r in res)
+    last = max(r["done_ms"] for _, r in res)
+    out = {"queries": len(res), "qps": round(len(res) / ((last - first) / 1e3), 2),
+           "wall_s": round(wall, 3)}
+    for name in ("all", "bfs", "sssp"):
+        lat = [r["done_ms"] - r["submit_ms"] for a, r in res if name in ("all", a)]
+        if lat:
+            out[f"{name}_p50_ms"] = round(pct(lat, 50), 3)
+            out[f"{name}_p99_ms"] = round(pct(lat, 99), 3)
+    return out, res
+
+
+def serving_bench(T, args):
+    """Resident R-MAT scale 25 / ef 16 with weights in [1, 64]: one query per
+    launch vs. 64-lane batches vs. the continuous-batching server."""
+    t0 = time.perf_counter()
+    g = T.Graph.rmat(args.serve_scale, 16, seed=2, symmetric=True, weights=True)
+    g.store_config(None, 1 << 16)
+    log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(3)
+    out = {"graph": {"scale": args.serve_scale, "edge_factor": 16, "vertices": g.n,
+                     "edges": g.m, "edge_bytes": 8, "resident": True}}
+    # the same 64 uniform random sources: one query per launch sequence
+    # (plain single-source kernels) vs. one 64-lane batch
+    b64 = rng.integers(0, g.n, 64).astype(np.uint32)
+    deg = np.diff(g.offsets()).astype(np.int64)
+    out["sources_isolated"] = int((deg[b64] == 0).sum())
+    for algo, fn in (("bfs", g.bfs), ("sssp", g.sssp)):
+        fn(int(b64[0]), out=False)
+        t = time.perf_counter()
+        for s in b64:
+            fn(int(s), out=False)
+        out[f"plain_{algo}_qps"] = round(64 / (time.perf_counter() - t), 2)
+        r = g.batch(algo, b64)
+        out[f"batch64_{algo}_qps"] = round(64 / (r["stats"].ms / 1e3), 2)
+        out[f"batch64_{algo}_levels"] = r["stats"].iterations
+        out[f"batch64_{algo}_vs_plain"] = round(out[f"batch64_{algo}_qps"] /
+                                                out[f"plain_{algo}_qps"], 2)
+    log(f"plain/batch: {out}")
+    # BFS-only stream: saturation, then Poisson at 70% of it
+    nb = args.serve_queries // 2
+    qb = [("bfs", int(s)) for s in rng.integers(0, g.n, nb)]
+    sat, _ = serve_stream(T, g, qb, None, 4)
+    pois, _ = serve_stream(T, g, qb, 0.7 * sat["qps"], 5)
+    out["serve_bfs_saturated"] = sat
+    out["serve_bfs_poisson70"] = pois
+    out["serve_bfs_speedup_vs_plain"] = round(sat["qps"] / out["plain_bfs_qps"], 2)
+    log(f"bfs stream: {sat} / {pois}")
+    # the mixed 50/50 BFS+SSSP stream of the configuration
+    qm = [("bfs" if rng.random() < 0.5 else "sssp", int(s))
+          for s in rng.integers(0, g.n, args.serve_queries)]
+    msat, _ = serve_stream(T, g, qm, None, 6)
+    mpois, _ = serve_stream(T, g, qm, 0.7 * msat["qps"], 7)
+    out["serve_mix_saturated"] = msat
+    out["serve_mix_poisson70"] = mpois
+    log(f"mix stream: {msat} / {mpois}")
+    g.close()
+    return out
+
+
+def step_summary(stats, wall, kernel):
+    """Throughput, mixed roofline and expansion-kernel link figures of K runs."""
+    exp_ms = sum(s.expand_ms for s in stats)
+    launches = sum(s.expand_launches for s in stats)
+    host_all = sum(s.host_bytes for s in stats)
+    hbm_all = sum(s.hbm_bytes for s in stats)
+    med = statistics.median(s.ms for s in stats)
+    t_roof = statistics.median(mixed_roofline_ms(s.hbm_bytes, s.host_bytes) for s in stats)
+    link = host_all / (exp_ms * 1e-3) / 1e9 if exp_ms else 0.0
+    return {
+        "edges_per_run": stats[0].edges_traversed,
+        "iterations": stats[0].iterations,
+        "ms_median": round(med, 3),
+        "gteps": round(sum(s.edges_traversed for s in stats) / wall / 1e9, 4),
+        "roofline": {"bound": "host-link", "kernel": kernel, "achieved": round(link, 2),
+                     "peak": LINK_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(link * 1e9 / LINK_PEAK, 4),
+                     "avg_launch_ms": round(exp_ms / launches, 4) if launches else None,
+                     "launches": launches,
+                     "mixed": {"t_roof_ms": round(t_roof, 3), "t_ms": round(med, 3),
+                               "frac": round(t_roof / med, 4)}},
+        "host_bytes_per_run": stats[0].host_bytes,
+        "hbm_bytes_per_run": stats[0].hbm_bytes,
+    }
+
+
+def sssp_bench(T, args):
+    """SSSP from a seeded random source with degree > 0 on R-MAT scale 26 /
+    ef 16, 8-byte {dst, weight} records, weights in [1, 64], 30% HBM budget."""
+    g = T.Graph.rmat(args.scale, 16, seed=1, symmetric=True, weights=True)
+    g.store_config(int(0.30 * g.m * 8), args.partition_edges)
+    deg = np.diff(g.offsets())
+    rng = np.random.default_rng(1)
+    src = int(rng.choice(np.nonzero(deg)[0]))
+    g.sssp(src, out=False)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(g.sssp(src, out=False)[1])
+    out = {"workload": f"sssp-rmat{args.scale}-ef16-w8-budget30pct", "source": src,
+           "vertices": g.n, "edges": g.m, "store": g.store_info()}
+    out.update(step_summary(stats, time.perf_counter() - t0, "k_expand<SSSP>"))
+    dist, _ = g.sssp(src)
+    if not args.no_cpu:
+        from oracle import oracle as O
+        off, dst, w, _ = g.export()
+        t = time.perf_counter()
+        cd = O.sssp_mt(O.CSR(off, dst, w), src)
+        ct = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(stats[0].edges_traversed / ct / 1e9, 4),
+                               "unit": "G processed edges/s", "cores": O.threads(),
+                               "kind": "port", "seconds": round(ct, 2),
+                               "sample": "1 frontier Bellman-Ford run (orc_sssp_mt)"}
+        out["cpu_matches_gpu"] = bool(np.array_equal(cd, dist))
+    g.close()
+    return out
+
+
+def pr_bench(T, args):
+    """PageRank push (CSR) and pull (CSC) on directed R-MAT scale 26 / ef 16,
+    25% HBM budget, fp32, stop at max |delta| < 1e-7 or 100 iterations;
+    normalised L1 against the FP64 fixpoint."""
+    out = {"workload": f"pr-rmat{args.scale}-ef16-directed-budget25pct"}
+    res = {}
+    for mode, transpose in (("push", False), ("pull", True)):
+        g = T.Graph.rmat(args.scale, 16, seed=1, symmetric=False, transpose=transpose)
+        g.store_config(int(0.25 * g.m * 4), args.partition_edges)
+        t0 = time.perf_counter()
+        r, st = g.pr(mode)
+        wall = time.perf_counter() - t0
+        s = step_summary([st], wall, f"k_expand<PR_{mode.upper()}>")
+        s["vertices"], s["edges"] = g.n, g.m
+        out[mode] = s
+        res[mode] = r
+        if transpose and not args.no_cpu:
+            from oracle import oracle as O
+            off, src, _, od = g.export()
+            t = time.perf_counter()
+            ex, it = O.pr_exact_mt(O.CSR(off, src, None, od), tol=1e-10, max_iter=300)
+            out["fp64_oracle"] = {"iterations": it, "seconds": round(time.perf_counter() - t, 1),
+                                  "cores": O.threads()}
+            for k in res:
+                out[k]["l1_vs_fp64"] = O.normalized_l1(res[k], ex)
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +244,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cc", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-scale", type=int, default=25)
+    ap.add_argument("--serve-queries", type=int, default=2000)
+    ap.add_argument("--workload", choices=["bfs", "sssp", "pr"], default="bfs",
+                    help="bfs: the headline line (BFS + CC + serving); sssp / pr: the "
+                         "SSSP and PageRank configurations, printed as their own line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -81,6 +262,15 @@ def main():
     import thievory_amd as T
     dev = T.device_info()
     log(f"device {dev}")
+    if args.serve_only:
+        print(json.dumps({"serving": serving_bench(T, args)}), flush=True)
+        return
+    if args.workload == "sssp":
+        print(json.dumps(sssp_bench(T, args)), flush=True)
+        return
+    if args.workload == "pr":
+        print(json.dumps(pr_bench(T, args)), flush=True)
+        return
     t0 = time.perf_counter()
     g = T.Graph.rmat(args.scale, args.edge_factor, seed=1, symmetric=True)
     gen_s = time.perf_counter() - t0
@@ -163,6 +353,9 @@ def main():
         out["cpu_baseline"] = cb
         out["cpu_matches_gpu"] = bool(np.array_equal(cpu_lv, lv))
         log(f"cpu baseline: {cb}")
+    g.close()
+    if not args.no_serve:
+        out["serving"] = serving_bench(T, args)
     print(json.dumps(out), flush=True)
 
 

@@ -484,6 +484,36 @@ ORC_EXPORT int orc_pr_exact(uint64_t n, const uint64_t *off,
   return it;
 }
 
+/* The same FP64 fixpoint, multithreaded, as a Jacobi pull over the CSC
+ * (in-offsets, in-sources, out-degrees); used where the serial form is too
+ * slow (full-size PageRank workloads).  Returns the iteration count. */
+ORC_EXPORT int orc_pr_exact_mt(uint64_t n, const uint64_t *ioff, const uint32_t *isrc,
+                               const uint32_t *outdeg, double alpha, double tol,
+                               int max_iter, double *rank) {
+  double *contrib = (double *)malloc((n ? n : 1) * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (long long v = 0; v < (long long)n; v++) rank[v] = 1.0 - alpha;
+  int it = 0;
+  for (; it < max_iter; it++) {
+#pragma omp parallel for schedule(static)
+    for (long long u = 0; u < (long long)n; u++)
+      contrib[u] = outdeg[u] ? rank[u] / (double)outdeg[u] : 0.0;
+    double md = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : md)
+    for (long long v = 0; v < (long long)n; v++) {
+      double s = 0;
+      for (uint64_t i = ioff[v]; i < ioff[v + 1]; i++) s += contrib[isrc[i]];
+      double nr = (1.0 - alpha) + alpha * s;
+      double dd = fabs(nr - rank[v]);
+      if (dd > md) md = dd;
+      rank[v] = nr;
+    }
+    if (md < tol) { it++; break; }
+  }
+  free(contrib);
+  return it;
+}
+
 /* PageRank pull, the reference's loop in FP64 (pr_kernels.cu:45-51,
  * 449-476; init rank = 1/n, graph.cuh:255-256): active v recomputes
  * sum over in-neighbours, stays active while |new-old| > tol.  Input is the

@@ -66,6 +66,9 @@ def lib():
         L.orc_pr_exact.restype = C.c_int
         L.orc_pr_exact.argtypes = [C.c_uint64, u64p, u32p, C.c_double, C.c_double,
                                    C.c_int, f64p]
+        L.orc_pr_exact_mt.restype = C.c_int
+        L.orc_pr_exact_mt.argtypes = [C.c_uint64, u64p, u32p, u32p, C.c_double, C.c_double,
+                                      C.c_int, f64p]
         L.orc_pr_pull_ref.restype = C.c_int
         L.orc_pr_pull_ref.argtypes = [C.c_uint64, u64p, u32p, u32p, C.c_double,
                                       C.c_double, C.c_int, f64p]
@@ -254,6 +257,13 @@ ALPHA_F32 = float(np.float32(0.85))
 def pr_exact(g, alpha=ALPHA_F32, tol=1e-13, max_iter=10000):
     out = np.zeros(g.n, np.float64)
     it = lib().orc_pr_exact(g.n, g.off, g.adj, alpha, tol, max_iter, out)
+    return out, it
+
+
+def pr_exact_mt(csc, alpha=ALPHA_F32, tol=1e-12, max_iter=10000):
+    """FP64 fixpoint from the CSC (multithreaded Jacobi)."""
+    out = np.zeros(csc.n, np.float64)
+    it = lib().orc_pr_exact_mt(csc.n, csc.off, csc.adj, csc.outdeg, alpha, tol, max_iter, out)
     return out, it
 
 

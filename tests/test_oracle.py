@@ -223,6 +223,8 @@ def test_pagerank_vs_sparse_solve():
     r = spla.spsolve((sp.identity(g.n) - alpha * M).tocsc(), np.full(g.n, 1 - alpha))
     ex, _ = O.pr_exact(g)
     assert np.max(np.abs(ex - r)) < 1e-9
+    exm, _ = O.pr_exact_mt(O.transpose(g), tol=1e-13)
+    assert np.max(np.abs(exm - r)) < 1e-9
     # the reference's gated loops converge to the same fixpoint
     push, _ = O.pr_push_ref(g, tol=1e-13, max_iter=100000)
     pull, _ = O.pr_pull_ref(O.transpose(g), tol=1e-13, max_iter=100000)

@@ -217,6 +217,11 @@ class Graph:
         return {"partitions": p.value, "device_partitions": pd.value,
                 "device_edges": de.value}
 
+    def offsets(self):
+        off = np.zeros(self.n + 1, np.uint64)
+        check(lib().thv_graph_export(self._h, _p(off), None, None, None))
+        return off
+
     def export(self):
         off = np.zeros(self.n + 1, np.uint64)
         dst = np.zeros(self.m, np.uint32)

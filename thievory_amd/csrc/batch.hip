@@ -117,12 +117,16 @@ __global__ void __launch_bounds__(MBS) k_ms_bfs(MsArgs a) {
   if (tid == 0 && s_lm) atomicOr(&a.acc[ACC_LM], s_lm);
 }
 
-// SSSP: one wave per edge, lane q relaxes query q; two edges in flight per
-// wave for memory-level parallelism.
-template <int EPT>
+// SSSP: lane q of a wave relaxes query q.  A wave owns 64 consecutive
+// work-space edges at a time: each lane loads one record (coalesced) and
+// ranks it once; the wave then walks the 64 edges, broadcasting each with
+// readlane, with the source and destination distance rows of 8 edges in
+// flight before the first compare.
+constexpr uint32_t SSSP_C = 1024;  // edges per chunk (4 waves x 4 x 64)
+
 __global__ void __launch_bounds__(MBS) k_ms_sssp(MsArgs a) {
-  constexpr uint32_t C = MBS * EPT, NW = C / 32;
-  constexpr uint32_t WAVES = MBS / 64;
+  constexpr uint32_t C = SSSP_C, NW = C / 32;
+  constexpr uint32_t WAVES = MBS / 64, PER_WAVE = C / WAVES;
   __shared__ uint32_t s_mask[NW], s_pre[NW];
   __shared__ unsigned long long s_base[C + 1];
   __shared__ unsigned long long s_m[C + 1];
@@ -130,6 +134,7 @@ __global__ void __launch_bounds__(MBS) k_ms_sssp(MsArgs a) {
   __shared__ unsigned long long s_lm;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t L = a.L;
+  const bool lane_ok = lane < L;
   if (tid == 0) s_lm = 0;
   const ChunkSrc cs = chunk_src(a.ctr, C);
   for (uint32_t c = blockIdx.x; c < cs.total; c += gridDim.x) {
@@ -153,38 +158,43 @@ __global__ void __launch_bounds__(MBS) k_ms_sssp(MsArgs a) {
     __syncthreads();
     chunk_prefix<NW>(s_mask, s_pre);
     __syncthreads();
-    for (uint32_t l0 = wv; l0 < ci.len; l0 += 2 * WAVES) {
-      uint32_t w[2], wt[2], v[2];
-      unsigned long long m[2];
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const uint32_t l = l0 + j * WAVES;
-        w[j] = 0xFFFFFFFFu;
-        m[j] = 0;
-        v[j] = 0;
-        wt[j] = 0;
-        if (l < ci.len) {
-          const uint32_t k = chunk_rank(s_mask, s_pre, l);
-          const uint2 r = *reinterpret_cast<const uint2 *>(s_base[k] + (ci.e0 + l) * 8);
-          w[j] = r.x;
-          wt[j] = r.y;
-          m[j] = s_m[k];
-          v[j] = s_v[k];
-        }
+    for (uint32_t g0 = wv * PER_WAVE; g0 < (wv + 1) * PER_WAVE && g0 < ci.len; g0 += 64) {
+      const uint32_t cnt = min(64u, ci.len - g0);
+      // lane j holds edge g0 + j: destination, weight, entry
+      uint32_t my_w = 0, my_wt = 0, my_k = 0;
+      if (lane < cnt) {
+        my_k = chunk_rank(s_mask, s_pre, g0 + lane);
+        const uint2 r = *reinterpret_cast<const uint2 *>(s_base[my_k] + (ci.e0 + g0 + lane) * 8);
+        my_w = r.x;
+        my_wt = r.y;
       }
+      for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
+        uint32_t w[8], dv[8], dw[8], wt[8];
+        bool on[8];
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        bool imp = false;
-        if (w[j] != 0xFFFFFFFFu && lane < L && ((m[j] >> lane) & 1ull)) {
-          const unsigned long long nd =
-              (unsigned long long)a.dist[(uint64_t)v[j] * L + lane] + wt[j];
-          uint32_t *dp = &a.dist[(uint64_t)w[j] * L + lane];
-          if (nd < *dp) imp = nd < atomicMin(dp, (uint32_t)nd);
+        for (int j = 0; j < 8; j++) {
+          const uint32_t jj = min(j0 + j, cnt - 1);
+          w[j] = __builtin_amdgcn_readlane(my_w, jj);
+          wt[j] = __builtin_amdgcn_readlane(my_wt, jj);
+          const uint32_t k = __builtin_amdgcn_readlane(my_k, jj);
+          on[j] = j0 + j < cnt && lane_ok && ((s_m[k] >> lane) & 1ull);
+          dv[j] = on[j] ? a.dist[(uint64_t)s_v[k] * L + lane] : 0u;
         }
-        const unsigned long long b = __ballot(imp);
-        if (b && lane == 0) {
-          if (atomicOr(&a.N[w[j]], b) == 0ull) atomicOr(&a.nb[w[j] >> 5], 1u << (w[j] & 31));
-          atomicOr(&s_lm, b);
+#pragma unroll
+        for (int j = 0; j < 8; j++) dw[j] = on[j] ? a.dist[(uint64_t)w[j] * L + lane] : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          bool imp = false;
+          if (on[j]) {
+            const unsigned long long nd = (unsigned long long)dv[j] + wt[j];
+            if (nd < dw[j])
+              imp = nd < atomicMin(&a.dist[(uint64_t)w[j] * L + lane], (uint32_t)nd);
+          }
+          const unsigned long long b = __ballot(imp);
+          if (b && lane == 0) {
+            if (atomicOr(&a.N[w[j]], b) == 0ull) atomicOr(&a.nb[w[j] >> 5], 1u << (w[j] & 31));
+            atomicOr(&s_lm, b);
+          }
         }
       }
     }
@@ -289,7 +299,7 @@ MultiEngine::MultiEngine(Graph &gr, int al, uint32_t lanes, bool kf)
   store_ensure(g);
   HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   const uint64_t n = g.n, words = bitmap_words(n);
-  qs.ensure(g, 256u * (uint32_t)g.shape.ept);
+  qs.ensure(g, chunk());
   for (int i = 0; i < 2; i++) {
     fb[i].alloc(words * 4);
     F[i].alloc((n + 1) * 8);
@@ -328,6 +338,10 @@ uint64_t MultiEngine::bytes_needed(const Graph &g, int algo, uint32_t lanes, boo
   if (algo == THV_ALGO_BFS) b += (n + 1) * 8 + (keep_full ? (uint64_t)lanes * n * 4 : 0);
   else b += (uint64_t)lanes * n * 4 + (keep_full ? n * 4 : 0);
   return b;
+}
+
+uint32_t MultiEngine::chunk() const {
+  return algo == THV_ALGO_SSSP ? SSSP_C : 256u * (uint32_t)g.shape.ept;
 }
 
 uint64_t MultiEngine::free_mask() const {
@@ -413,7 +427,7 @@ void MultiEngine::launch_level() {
     for (uint32_t q = 0; q < MAXL; q++) a.voff[q] = voff[q];
     const int grid = g.shape.blocks_per_cu * g.cus;
     const bool r8 = g.rec == 8;
-    if (qs.C != 256u * (uint32_t)g.shape.ept) fail(THV_ESTATE, "queue chunk size mismatch");
+    if (qs.C != chunk()) fail(THV_ESTATE, "queue chunk size mismatch");
     if (algo == THV_ALGO_BFS) {
       if (g.shape.ept == 4) {
         if (r8) hipLaunchKernelGGL((k_ms_bfs<8, 4>), dim3(grid), dim3(MBS), 0, stream, a);
@@ -423,8 +437,8 @@ void MultiEngine::launch_level() {
         else hipLaunchKernelGGL((k_ms_bfs<4, 8>), dim3(grid), dim3(MBS), 0, stream, a);
       }
     } else {
-      if (g.shape.ept == 4) hipLaunchKernelGGL((k_ms_sssp<4>), dim3(grid), dim3(MBS), 0, stream, a);
-      else hipLaunchKernelGGL((k_ms_sssp<8>), dim3(grid), dim3(MBS), 0, stream, a);
+      // 20 KB of LDS per block: fill every CU with 7 resident blocks
+      hipLaunchKernelGGL(k_ms_sssp, dim3(7 * g.cus), dim3(MBS), 0, stream, a);
     }
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(&h_ctr[level & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, stream));

@@ -219,21 +219,29 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
     p = q;
   }
 
-  // Per-vertex first-record address.
+  // Per-vertex first-record address, residency bit and partition id.
   std::vector<uint64_t> eptr(g.n);
+  std::vector<uint32_t> rbit(bitmap_words(g.n), 0u), vpart(g.n);
   const uint64_t slab_base = (uint64_t)(uintptr_t)s.slab.p;
   const uint64_t host_base = (uint64_t)(uintptr_t)g.h_edges.dev;
   for (uint32_t p = 0; p < s.P; p++) {
     uint64_t pe0 = g.h_off[s.pv[p]];
     for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) {
-      if (s.pres[p])
+      vpart[v] = p;
+      if (s.pres[p]) {
         eptr[v] = slab_base + (slab_off[p] + (g.h_off[v] - pe0)) * rec;
-      else
+        rbit[v >> 5] |= 1u << (v & 31);
+      } else {
         eptr[v] = host_base + g.h_off[v] * rec;
+      }
     }
   }
   s.d_eptr.alloc(g.n * 8);
   if (g.n) HIPC(hipMemcpy(s.d_eptr.p, eptr.data(), g.n * 8, hipMemcpyHostToDevice));
+  s.d_rbit.alloc(rbit.size() * 4);
+  HIPC(hipMemcpy(s.d_rbit.p, rbit.data(), rbit.size() * 4, hipMemcpyHostToDevice));
+  s.d_vpart.alloc(g.n * 4);
+  if (g.n) HIPC(hipMemcpy(s.d_vpart.p, vpart.data(), g.n * 4, hipMemcpyHostToDevice));
   s.d_pv.alloc((s.P + 1) * 4);
   HIPC(hipMemcpy(s.d_pv.p, s.pv.data(), (s.P + 1) * 4, hipMemcpyHostToDevice));
   s.d_pres.alloc(s.P ? s.P : 1);
@@ -242,13 +250,18 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
 }
 
 void QueueSet::ensure(const Graph &g, uint32_t chunk) {
-  if (P != g.store.P || !pcnt.p) {
-    const uint64_t np = g.store.P + 1;
-    pcnt.alloc(np * 4);
-    pdeg.alloc(np * 8);
-    pbc.alloc(np * 4);
-    pbe.alloc(np * 8);
+  if (P != g.store.P || !ptouch.p) {
+    ptouch.alloc((uint64_t)(g.store.P + 1) * 4);
+    HIPC(hipMemset(ptouch.p, 0, (uint64_t)(g.store.P + 1) * 4));
+    stamp = 0;
     P = g.store.P;
+  }
+  const uint64_t tiles = g.n / TILE + 1;
+  if (tcnt.bytes < tiles * 2 * 4) {
+    tcnt.alloc(tiles * 2 * 4);
+    tdeg.alloc(tiles * 2 * 8);
+    tbc.alloc(tiles * 2 * 4);
+    tbe.alloc(tiles * 2 * 8);
   }
   if (n == g.n && m == g.m && C == chunk && ctr.p) return;
   for (int q = 0; q < 2; q++) {

@@ -80,6 +80,8 @@ struct Store {
   std::vector<uint8_t> pres;         // P residency (1 = HBM)
   DevMem slab;                       // resident records
   DevMem d_pv, d_pres, d_eptr;
+  DevMem d_rbit;                     // n bits: vertex's edges are in HBM
+  DevMem d_vpart;                    // n x u32: partition of each vertex
 };
 
 // ---- workspace (per graph, allocated on first use) --------------------------
@@ -99,13 +101,18 @@ struct Graph;
 // Frontier queues of one traversal engine: filled by the compaction kernels,
 // consumed by an expansion kernel.  Engines that run concurrently on
 // different streams own separate QueueSets.
+constexpr uint32_t TILE = 4096;  // vertices per compaction tile
+
 struct QueueSet {
   uint64_t n = 0, m = 0;
   uint32_t C = 0;             // edges per expansion chunk
-  uint32_t P = 0;             // partitions the per-partition arrays cover
+  uint32_t P = 0;             // partitions the touch stamps cover
+  uint32_t stamp = 0;         // compaction call counter (touch stamps)
   DevMem qv[2], qe[2], qc[2]; // queue vertex ids, edge prefixes, chunk starts
   DevMem ctr;                 // LevelCtr[2]
-  DevMem pcnt, pdeg, pbc, pbe; // per partition: active count / edges, bases
+  DevMem tcnt, tdeg;          // per tile and queue: active vertices / edges
+  DevMem tbc, tbe;            // per tile and queue: exclusive bases
+  DevMem ptouch;              // per partition: stamp of the last touching call
   void ensure(const Graph &g, uint32_t C);
 };
 

@@ -68,6 +68,7 @@ struct MultiEngine {
   MultiEngine &operator=(const MultiEngine &) = delete;
 
   static uint64_t bytes_needed(const Graph &g, int algo, uint32_t lanes, bool keep_full);
+  uint32_t chunk() const;                  // edges per expansion chunk
   uint64_t free_mask() const;
   uint32_t free_lanes() const { return (uint32_t)__builtin_popcountll(free_mask()); }
   int admit(uint32_t src, uint64_t tag);   // -> lane, or -1 when none is free
