@@ -68,6 +68,11 @@ typedef struct {
   uint32_t expand_launches;         /* edge-expansion kernel launches      */
   double ms;                /* first launch -> empty frontier (wall)      */
   double expand_ms;         /* time inside edge-expansion kernels (events)*/
+  uint64_t edges_examined;  /* edge records actually read (bottom-up BFS
+                               levels stop early, so this can be smaller
+                               than edges_traversed)                       */
+  uint32_t bottom_up_levels;/* BFS levels run bottom-up                    */
+  uint32_t pad;
 } thv_stats;
 
 /* ---- graph lifetime --------------------------------------------------- */
@@ -99,6 +104,25 @@ int thv_graph_rmat(int scale, int edge_factor, double a, double b, double c,
 
 int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
                    int *edge_bytes, int *is_csc);
+
+/* Per-graph options (no reference counterpart).
+ *   THV_OPT_SYMMETRIC   1: every edge u->v has v->u (set by thv_graph_rmat
+ *                       for symmetric graphs; declare it for such files)
+ *   THV_OPT_DIRECTION   1 (default): BFS on a symmetric graph switches to
+ *                       bottom-up levels when the frontier is large
+ *                       (direction-optimising BFS, Beamer et al. SC'12);
+ *                       0: top-down only, as in the reference
+ *   THV_OPT_DO_ALPHA    top-down -> bottom-up when frontier edges exceed
+ *                       unvisited edges / alpha (default 14)
+ *   THV_OPT_DO_BETA     bottom-up -> top-down when the frontier holds fewer
+ *                       than n / beta vertices (default 24)
+ * Values are the same for BFS results either way (levels are unique). */
+#define THV_OPT_SYMMETRIC 1
+#define THV_OPT_DIRECTION 2
+#define THV_OPT_DO_ALPHA 3
+#define THV_OPT_DO_BETA 4
+int thv_graph_set_option(thv_graph *g, int option, double value);
+int thv_graph_get_option(const thv_graph *g, int option, double *value);
 
 /* Copies the CSR arrays to host buffers (any may be NULL). */
 int thv_graph_export(const thv_graph *g, uint64_t *off, uint32_t *dst,

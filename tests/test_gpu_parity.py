@@ -114,6 +114,49 @@ def test_bfs_cc_sssp_split_store(rmat16, frac, part):
     assert np.array_equal(g.cc()[0], O.cc(og))
 
 
+@pytest.mark.parametrize("frac", [1.0, 0.25, 0.0])
+def test_bfs_direction_optimizing(rmat16, frac):
+    """Bottom-up levels on a symmetric graph give the same levels; edges
+    examined drop below the traversed count."""
+    g, og = rmat16
+    _say("case: direction-optimising bfs", frac)
+    g.store_config(int(frac * g.m * g.edge_bytes), 4096)
+    assert g.get_option("symmetric") == 1.0 and g.get_option("direction") == 1.0
+    deg = og.degrees()
+    for s in (1, 7, 12345, 65535):
+        ref = O.bfs(og, s)
+        lv, st = g.bfs(s)
+        assert np.array_equal(lv, ref), s
+        assert st.edges_traversed == int(deg[ref != og.n + 1].sum())
+        assert st.edges_examined <= st.edges_traversed
+        if s == 1:
+            assert st.bottom_up_levels > 0
+            assert st.edges_examined < st.edges_traversed
+    g.set_option("direction", 0)
+    lv, st = g.bfs(1)
+    assert st.bottom_up_levels == 0 and np.array_equal(lv, O.bfs(og, 1))
+    g.set_option("do_alpha", 2.0)   # switch earlier: still exact
+    g.set_option("direction", 1)
+    assert np.array_equal(g.bfs(7)[0], O.bfs(og, 7))
+    g.set_option("do_alpha", 14.0)
+
+
+def test_symmetric_hint_on_arrays():
+    og = O.rmat(12, 16, seed=11)
+    g = T.Graph.from_arrays(og.off, og.adj)
+    assert g.get_option("symmetric") == 0.0
+    g.set_option("symmetric", 1)
+    g.store_config(int(0.2 * g.m * 4), 512)
+    for s in (0, 1, 100):
+        lv, st = g.bfs(s)
+        assert np.array_equal(lv, O.bfs(og, s))
+    with pytest.raises(T.ThvError):
+        g.set_option("do_beta", -1)
+    gc = T.Graph.rmat(10, 8, symmetric=False, transpose=True)
+    with pytest.raises(T.ThvError):
+        gc.set_option("symmetric", 1)
+
+
 def test_pr_push_pull_rmat():
     _say("case: rmat14 directed pagerank")
     og = O.rmat(14, 16, seed=2, symmetric=False)

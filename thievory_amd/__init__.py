@@ -26,6 +26,7 @@ CLI_PATH = os.path.join(HERE, "_lib", "thv_main")
 FMT = {"csr": 0, "csc": 1, "wcsr": 2}
 ALGO = {"bfs": 0, "cc": 1, "sssp": 2, "pr": 3}
 PR_MODE = {"push": 0, "pull": 1}
+OPTIONS = {"symmetric": 1, "direction": 2, "do_alpha": 3, "do_beta": 4}
 ERRORS = {0: "OK", -1: "EINVAL", -2: "EIO", -3: "ENOMEM", -4: "EHIP", -5: "EFORMAT",
           -6: "EBUSY", -7: "ENOTSUP", -8: "ESTATE"}
 
@@ -44,10 +45,12 @@ class Stats(C.Structure):
                 ("reached", C.c_uint64), ("iterations", C.c_uint32),
                 ("host_partitions_touched", C.c_uint32),
                 ("host_partitions_skipped", C.c_uint32), ("expand_launches", C.c_uint32),
-                ("ms", C.c_double), ("expand_ms", C.c_double)]
+                ("ms", C.c_double), ("expand_ms", C.c_double),
+                ("edges_examined", C.c_uint64), ("bottom_up_levels", C.c_uint32),
+                ("pad", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class ServeCfg(C.Structure):
@@ -89,6 +92,8 @@ def lib():
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "thv_graph_export": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "thv_graph_close": (None, [_vp]),
+        "thv_graph_set_option": (C.c_int, [_vp, C.c_int, C.c_double]),
+        "thv_graph_get_option": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
         "thv_store_config": (C.c_int, [_vp, C.c_uint64, C.c_uint32]),
         "thv_store_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint64)]),
@@ -216,6 +221,14 @@ class Graph:
         check(lib().thv_store_info(self._h, C.byref(p), C.byref(pd), C.byref(de)))
         return {"partitions": p.value, "device_partitions": pd.value,
                 "device_edges": de.value}
+
+    def set_option(self, name, value):
+        check(lib().thv_graph_set_option(self._h, OPTIONS[name], float(value)))
+
+    def get_option(self, name):
+        v = C.c_double()
+        check(lib().thv_graph_get_option(self._h, OPTIONS[name], C.byref(v)))
+        return v.value
 
     def offsets(self):
         off = np.zeros(self.n + 1, np.uint64)

@@ -32,7 +32,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Wall", "-Wno-unused-result", "-I", INCLUDE]
 
 LIB_SOURCES = ["graph.cpp", "io.cpp", "abi.cpp", "frontier.hip", "expand.hip",
-               "algos.hip", "rmat.hip", "batch.hip", "serve.cpp", "autotune.cpp"]
+               "algos.hip", "rmat.hip", "batch.hip", "serve.cpp", "autotune.cpp",
+               "bottomup.hip"]
 CLI_SOURCES = ["cli.cpp"]
 
 

@@ -70,6 +70,44 @@ int thv_graph_rmat(int scale, int edge_factor, double a, double b, double c, uin
   });
 }
 
+int thv_graph_set_option(thv_graph *gh, int option, double value) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    switch (option) {
+      case THV_OPT_SYMMETRIC:
+        if (g->csc && value != 0.0) fail(THV_EINVAL, "a CSC graph holds in-edges only");
+        g->sym = value != 0.0;
+        break;
+      case THV_OPT_DIRECTION: g->dir_opt = value != 0.0; break;
+      case THV_OPT_DO_ALPHA:
+        NEED(value > 0.0);
+        g->do_alpha = value;
+        break;
+      case THV_OPT_DO_BETA:
+        NEED(value > 0.0);
+        g->do_beta = value;
+        break;
+      default: fail(THV_EINVAL, "unknown option");
+    }
+  });
+}
+
+int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
+  return guarded([&] {
+    NEED(gh && value);
+    const Graph *g = reinterpret_cast<const Graph *>(gh);
+    switch (option) {
+      case THV_OPT_SYMMETRIC: *value = g->sym ? 1.0 : 0.0; break;
+      case THV_OPT_DIRECTION: *value = g->dir_opt ? 1.0 : 0.0; break;
+      case THV_OPT_DO_ALPHA: *value = g->do_alpha; break;
+      case THV_OPT_DO_BETA: *value = g->do_beta; break;
+      default: fail(THV_EINVAL, "unknown option");
+    }
+  });
+}
+
 int thv_graph_info(const thv_graph *gh, uint64_t *n, uint64_t *m, int *edge_bytes, int *is_csc) {
   return guarded([&] {
     NEED(gh);

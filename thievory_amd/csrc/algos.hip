@@ -148,6 +148,7 @@ static void stats_level(Graph &g, thv_stats *st, const LevelCtr &h) {
   const uint64_t words = g.n / 32 + 2;
   const uint64_t e = h.qw[0] + h.qw[1];
   st->edges_traversed += e;
+  st->edges_examined += e;
   st->host_bytes += h.qw[0] * rec;
   // device-resident records, one 4-byte state access per edge, 12 bytes
   // (degree, first-record address half, own value) per active vertex, and
@@ -224,6 +225,104 @@ static void check_src(Graph &g, uint32_t src) {
                          std::to_string(g.n) + ")");
 }
 
+// Direction-optimising BFS on a symmetric graph (Beamer et al., SC'12).
+// Each level first counts the frontier (vertices n_f, edges m_f); with
+// m_u the edges of still-unvisited vertices, the level runs bottom-up when
+// m_f > m_u / alpha and returns to top-down once n_f < n / beta and the
+// frontier shrinks.  Top-down levels are the ordinary compaction +
+// expansion; bottom-up levels compact the unvisited set (inverted visited
+// bitmap) and let each vertex search its own list for a frontier parent.
+// edges_traversed keeps the algorithm-independent count (sum of degrees of
+// reached vertices); edges_examined counts the records actually read.
+static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t words = bitmap_words(g.n);
+  ExpandArgs a = base_args(g);
+  LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
+  const uint64_t rec = (uint64_t)g.rec;
+  int cur = 0;
+  bool bu = false;
+  uint64_t visited_edges = 0, prev_nf = 0;
+  double expand_ms = 0;
+  uint32_t launches = 0;
+  const uint64_t t0 = now_ns();
+  for (uint32_t L = 0;; L++) {
+    LevelCtr *c = d_ctr + (L & 1);
+    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_COUNT_ONLY);
+    HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const LevelCtr h = w.h_ctr[L & 1];
+    const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
+    if (nf == 0) break;
+    visited_edges += mf;
+    const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
+    if (!bu && (double)mf > (double)mu / g.do_alpha) bu = true;
+    else if (bu && (double)nf < (double)g.n / g.do_beta && nf < prev_nf) bu = false;
+    prev_nf = nf;
+    HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
+    if (!bu) {
+      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_WRITE_ONLY);
+      a.ctr = c;
+      a.next = w.front[cur ^ 1].as<uint32_t>();
+      a.level_value = L + 2;
+      HIPC(hipEventRecord(w.ev[0], s));
+      launch_expand(g, A_BFS, a, s);
+      HIPC(hipEventRecord(w.ev[1], s));
+    } else {
+      LevelCtr *cb = d_ctr + 2;
+      launch_compact(g, w.qs, w.vis.as<uint32_t>(), cb, s, CMP_INVERT | CMP_NO_CHUNKS);
+      BottomUpArgs b;
+      b.qv[0] = w.qs.qv[0].as<uint32_t>();
+      b.qv[1] = w.qs.qv[1].as<uint32_t>();
+      b.ctr = cb;
+      b.eptr = g.store.d_eptr.as<uint64_t>();
+      b.deg = g.d_deg.as<uint32_t>();
+      b.front = w.front[cur].as<uint32_t>();
+      b.vis = w.vis.as<uint32_t>();
+      b.next = w.front[cur ^ 1].as<uint32_t>();
+      b.val = w.val.as<uint32_t>();
+      b.rd_ctr = cb;
+      b.level_value = L + 2;
+      HIPC(hipEventRecord(w.ev[0], s));
+      launch_bottom_up(g, b, s);
+      HIPC(hipEventRecord(w.ev[1], s));
+      HIPC(hipMemcpyAsync(&w.h_ctr[2], cb, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    expand_ms += ms;
+    launches++;
+    if (st) {
+      st->edges_traversed += mf;
+      st->iterations++;
+      if (!bu) {
+        st->edges_examined += mf;
+        st->host_bytes += h.qw[0] * rec;
+        st->hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
+        st->host_partitions_touched += h.touched[0];
+      } else {
+        const LevelCtr &hb = w.h_ctr[2];
+        st->edges_examined += hb.rd[0] + hb.rd[1];
+        st->host_bytes += hb.rd[0] * rec;
+        // records in HBM, one frontier-bit probe per record, 12 bytes per
+        // unvisited vertex, and the visited / frontier bitmaps
+        st->hbm_bytes += hb.rd[1] * rec + 4 * (hb.rd[0] + hb.rd[1]) +
+                         12 * (hb.qn[0] + hb.qn[1]) + 12 * words;
+        st->host_partitions_touched += hb.touched[0];
+        st->bottom_up_levels++;
+      }
+    }
+    cur ^= 1;
+  }
+  if (st) {
+    st->ms = (now_ns() - t0) * 1e-6;
+    st->expand_ms = expand_ms;
+    st->expand_launches = launches;
+  }
+}
+
 void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   check_src(g, src);
   workspace_ensure(g);
@@ -238,7 +337,8 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.vis.as<uint32_t>(), src);
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
   HIPC(hipStreamSynchronize(s));
-  value_loop(g, A_BFS, st);
+  if (g.sym && g.dir_opt) bfs_direction_optimizing(g, st);
+  else value_loop(g, A_BFS, st);
   finish_values(g, out, st, (uint32_t)(g.n + 1));
 }
 

@@ -42,6 +42,8 @@ struct CompactArgs {
   uint32_t stamp;
   LevelCtr *ctr;
   uint32_t C;
+  uint32_t invert;     // select vertices whose bit is clear
+  uint32_t no_chunks;  // skip the chunk-start table
 };
 
 // Lane's vertex: active (frontier bit, degree > 0), residency queue, degree.
@@ -50,7 +52,7 @@ __device__ __forceinline__ void vertex_state(const CompactArgs &a, uint64_t v, b
   act = false;
   q = Q_HOST;
   d = 0;
-  if (v < a.n && ((a.front[v >> 5] >> (v & 31)) & 1u)) {
+  if (v < a.n && (((a.front[v >> 5] >> (v & 31)) & 1u) ^ a.invert)) {
     d = a.deg[v];
     act = d != 0;
     if (act && ((a.rbit[v >> 5] >> (v & 31)) & 1u)) q = Q_DEV;
@@ -129,6 +131,8 @@ __global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
     a.ctr->qw[1] = carry[3];
     a.ctr->touched[0] = 0;
     a.ctr->touched[1] = 0;
+    a.ctr->rd[0] = 0;
+    a.ctr->rd[1] = 0;
   }
 }
 
@@ -190,9 +194,11 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
         const unsigned long long e0 = eb[q] + (q == Q_HOST ? ih - xh : id - xd);
         a.qv[q][idx] = (uint32_t)v;
         a.qe[q][idx] = e0;
-        uint32_t *qc = a.qc[q];
-        for (unsigned long long ch = (e0 + a.C - 1) / a.C; ch * a.C < e0 + d; ch++)
-          qc[ch] = (uint32_t)idx;
+        if (!a.no_chunks) {
+          uint32_t *qc = a.qc[q];
+          for (unsigned long long ch = (e0 + a.C - 1) / a.C; ch * a.C < e0 + d; ch++)
+            qc[ch] = (uint32_t)idx;
+        }
       }
       // partitions touched: the first lane of every run of equal partition
       // ids stamps it; a stamp that changes counts the partition once
@@ -213,7 +219,7 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
 }  // namespace
 
 void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
-                    hipStream_t s) {
+                    hipStream_t s, int mode) {
   Store &st = g.store;
   CompactArgs a;
   memset(&a, 0, sizeof a);
@@ -240,10 +246,17 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   }
   a.ctr = ctr;
   a.C = w.C;
+  a.invert = (mode & CMP_INVERT) ? 1u : 0u;
+  a.no_chunks = (mode & CMP_NO_CHUNKS) ? 1u : 0u;
   const uint32_t grid = a.ntiles < 65535u ? a.ntiles : 65535u;
-  hipLaunchKernelGGL(k_tcount, dim3(grid), dim3(CBS), 0, s, a);
-  hipLaunchKernelGGL(k_tscan, dim3(1), dim3(SBS), 0, s, a);
-  hipLaunchKernelGGL(k_twrite, dim3(grid), dim3(CBS), 0, s, a);
+  // WRITE_ONLY continues a COUNT_ONLY call on the same bitmap and counters:
+  // the tile bases and totals are still in place
+  if (!(mode & CMP_WRITE_ONLY)) {
+    hipLaunchKernelGGL(k_tcount, dim3(grid), dim3(CBS), 0, s, a);
+    hipLaunchKernelGGL(k_tscan, dim3(1), dim3(SBS), 0, s, a);
+  }
+  if (!(mode & CMP_COUNT_ONLY))
+    hipLaunchKernelGGL(k_twrite, dim3(grid), dim3(CBS), 0, s, a);
   HIPC(hipGetLastError());
 }
 

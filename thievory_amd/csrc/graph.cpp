@@ -269,7 +269,7 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
     qe[q].alloc((g.n + 1) * 8);
     qc[q].alloc((g.m / chunk + 2) * 4);
   }
-  ctr.alloc(2 * sizeof(LevelCtr));
+  ctr.alloc(CTR_SLOTS * sizeof(LevelCtr));
   n = g.n;
   m = g.m;
   C = chunk;
@@ -285,7 +285,8 @@ void workspace_ensure(Graph &g) {
   w.vis.alloc(words * 4);
   w.aux.alloc(8 * sizeof(unsigned long long));
   w.val.alloc((g.n + 1) * 4);
-  if (!w.h_ctr) HIPC(hipHostMalloc((void **)&w.h_ctr, 2 * sizeof(LevelCtr), hipHostMallocDefault));
+  if (!w.h_ctr)
+    HIPC(hipHostMalloc((void **)&w.h_ctr, CTR_SLOTS * sizeof(LevelCtr), hipHostMallocDefault));
   if (!w.h_aux)
     HIPC(hipHostMalloc((void **)&w.h_aux, 8 * sizeof(unsigned long long), hipHostMallocDefault));
   for (auto &e : w.ev)

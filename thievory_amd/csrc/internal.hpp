@@ -94,7 +94,10 @@ struct LevelCtr {
   unsigned long long qw[2];   // queue edges
   unsigned int touched[2];    // partitions with >=1 active vertex
   unsigned int pad[2];
+  unsigned long long rd[2];   // bottom-up BFS: edge records read (host, HBM)
 };
+
+enum { CTR_SLOTS = 3 };       // [0], [1]: alternating levels; [2]: bottom-up queue
 
 struct Graph;
 
@@ -156,6 +159,9 @@ struct Graph {
   int cus = 256;
   std::mutex mu;              // serialises algorithm calls on one handle
   std::atomic<int> servers{0};  // running servers (store must stay fixed)
+  bool sym = false;           // every edge has its reverse
+  bool dir_opt = true;        // direction-optimising BFS when sym
+  double do_alpha = 14.0, do_beta = 24.0;
   ~Graph();
 };
 
@@ -186,8 +192,24 @@ struct ExpandArgs {
   int rec;
 };
 
+// Compaction modes: INVERT takes the vertices whose bit is CLEAR (the
+// unvisited set of a bottom-up BFS level), NO_CHUNKS skips the chunk-start
+// table, COUNT_ONLY stops after the totals.
+enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8 };
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
-                    hipStream_t s);
+                    hipStream_t s, int mode = 0);
+
+struct BottomUpArgs {
+  const uint32_t *qv[2];
+  const LevelCtr *ctr;        // queue of unvisited vertices (qn)
+  const uint64_t *eptr;
+  const uint32_t *deg;
+  const uint32_t *front;      // current frontier
+  uint32_t *vis, *next, *val;
+  LevelCtr *rd_ctr;           // receives rd[2]
+  uint32_t level_value;
+};
+void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
 ExpandArgs expand_args(const Graph &g, const QueueSet &qs);
 inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);

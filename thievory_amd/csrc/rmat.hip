@@ -121,6 +121,7 @@ Graph *rmat_generate(int scale, int edge_factor, double a, double b, double c, u
   g->n = n;
   g->rec = weights ? 8 : 4;
   g->csc = transpose;
+  g->sym = symmetric;
 
   RmatParams p;
   p.scale = scale;
