@@ -73,6 +73,11 @@ typedef struct {
                                than edges_traversed)                       */
   uint32_t bottom_up_levels;/* BFS levels run bottom-up                    */
   uint32_t pad;
+  uint64_t topdown_host_bytes; /* host_bytes / hbm_bytes a top-down-only    */
+  uint64_t topdown_hbm_bytes;  /* traversal of the same levels would need   */
+  double bottom_up_ms;         /* time inside bottom-up kernels (events);
+                                  expand_ms counts the expansion kernel only */
+  uint64_t bottom_up_host_bytes; /* part of host_bytes read bottom-up       */
 } thv_stats;
 
 /* ---- graph lifetime --------------------------------------------------- */
@@ -112,15 +117,23 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       bottom-up levels when the frontier is large
  *                       (direction-optimising BFS, Beamer et al. SC'12);
  *                       0: top-down only, as in the reference
- *   THV_OPT_DO_ALPHA    top-down -> bottom-up when frontier edges exceed
- *                       unvisited edges / alpha (default 14)
- *   THV_OPT_DO_BETA     bottom-up -> top-down when the frontier holds fewer
- *                       than n / beta vertices (default 24)
- * Values are the same for BFS results either way (levels are unique). */
+ *   THV_OPT_DO_ALPHA    all edges in HBM: top-down -> bottom-up when
+ *                       frontier edges exceed unvisited edges / alpha
+ *                       (default 14)
+ *   THV_OPT_DO_BETA     all edges in HBM: bottom-up -> top-down when the
+ *                       frontier holds fewer than n / beta vertices (24)
+ *   THV_OPT_DO_VERTEX_NS  split store: a level runs bottom-up when
+ *                       (unvisited host vertices x this many ns) is less
+ *                       than (frontier host edge bytes / link rate);
+ *                       default 1.5 (measured on MI355X)
+ *   THV_OPT_DO_LINK_GBPS  the link rate of that rule, default 55
+ * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
 #define THV_OPT_DO_ALPHA 3
 #define THV_OPT_DO_BETA 4
+#define THV_OPT_DO_VERTEX_NS 5
+#define THV_OPT_DO_LINK_GBPS 6
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

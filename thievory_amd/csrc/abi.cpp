@@ -89,6 +89,14 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
         NEED(value > 0.0);
         g->do_beta = value;
         break;
+      case THV_OPT_DO_VERTEX_NS:
+        NEED(value >= 0.0);
+        g->do_vertex_ns = value;
+        break;
+      case THV_OPT_DO_LINK_GBPS:
+        NEED(value > 0.0);
+        g->do_link_gbps = value;
+        break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });
@@ -103,6 +111,8 @@ int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
       case THV_OPT_DIRECTION: *value = g->dir_opt ? 1.0 : 0.0; break;
       case THV_OPT_DO_ALPHA: *value = g->do_alpha; break;
       case THV_OPT_DO_BETA: *value = g->do_beta; break;
+      case THV_OPT_DO_VERTEX_NS: *value = g->do_vertex_ns; break;
+      case THV_OPT_DO_LINK_GBPS: *value = g->do_link_gbps; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });

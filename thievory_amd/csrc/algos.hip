@@ -8,6 +8,7 @@
 // Initial values follow CSR::InitFrontierNValues (src/graph.cuh:229-268).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 
 #include "internal.hpp"
 #include "scan.hpp"
@@ -154,6 +155,8 @@ static void stats_level(Graph &g, thv_stats *st, const LevelCtr &h) {
   // (degree, first-record address half, own value) per active vertex, and
   // the frontier bitmap read + cleared once per level.
   st->hbm_bytes += h.qw[1] * rec + 4 * e + 12 * (h.qn[0] + h.qn[1]) + 8 * words;
+  st->topdown_host_bytes += h.qw[0] * rec;
+  st->topdown_hbm_bytes += h.qw[1] * rec + 4 * e + 12 * (h.qn[0] + h.qn[1]) + 8 * words;
   const uint32_t host_parts = g.store.P - g.store.Pdev;
   st->host_partitions_touched += h.touched[0];
   st->host_partitions_skipped += host_parts - std::min(host_parts, h.touched[0]);
@@ -243,8 +246,10 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const uint64_t rec = (uint64_t)g.rec;
   int cur = 0;
   bool bu = false;
-  uint64_t visited_edges = 0, prev_nf = 0;
-  double expand_ms = 0;
+  const bool trace = getenv("THV_TRACE") != nullptr;
+  const bool host_share = g.store.host_vertices > 0;
+  uint64_t visited_edges = 0, visited_host = 0, prev_nf = 0;
+  double expand_ms = 0, bottom_up_ms = 0;
   uint32_t launches = 0;
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
@@ -256,9 +261,21 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
     if (nf == 0) break;
     visited_edges += mf;
+    visited_host += h.qn[0];
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
-    if (!bu && (double)mf > (double)mu / g.do_alpha) bu = true;
-    else if (bu && (double)nf < (double)g.n / g.do_beta && nf < prev_nf) bu = false;
+    if (host_share) {
+      // split store: a top-down level streams the frontier's host edges
+      // over the link; a bottom-up level pays one small zero-copy read per
+      // unvisited host vertex (request-rate bound)
+      const uint64_t unv_host = g.store.host_vertices - std::min(g.store.host_vertices, visited_host);
+      const double td = (double)h.qw[0] * rec / (g.do_link_gbps * 1e9);
+      const double bu_cost = (double)unv_host * g.do_vertex_ns * 1e-9;
+      bu = bu_cost < td;
+    } else {
+      // everything in HBM: the edge-ratio rule of Beamer et al.
+      if (!bu && (double)mf > (double)mu / g.do_alpha) bu = true;
+      else if (bu && (double)nf < (double)g.n / g.do_beta && nf < prev_nf) bu = false;
+    }
     prev_nf = nf;
     HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
     if (!bu) {
@@ -292,11 +309,26 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     HIPC(hipStreamSynchronize(s));
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
-    expand_ms += ms;
-    launches++;
+    if (bu) {
+      bottom_up_ms += ms;
+    } else {
+      expand_ms += ms;
+      launches++;
+    }
+    if (trace) {
+      const LevelCtr &hb = w.h_ctr[2];
+      fprintf(stderr,
+              "[thv] bfs L=%u %s nf=%llu mf=%llu front_host_edges=%llu unvisited=%llu "
+              "read_host=%llu read_hbm=%llu kernel_ms=%.3f\n",
+              L, bu ? "bottom-up" : "top-down", (unsigned long long)nf,
+              (unsigned long long)mf, h.qw[0], bu ? hb.qn[0] + hb.qn[1] : 0ull,
+              bu ? hb.rd[0] : h.qw[0], bu ? hb.rd[1] : h.qw[1], ms);
+    }
     if (st) {
       st->edges_traversed += mf;
       st->iterations++;
+      st->topdown_host_bytes += h.qw[0] * rec;
+      st->topdown_hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
       if (!bu) {
         st->edges_examined += mf;
         st->host_bytes += h.qw[0] * rec;
@@ -306,6 +338,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         const LevelCtr &hb = w.h_ctr[2];
         st->edges_examined += hb.rd[0] + hb.rd[1];
         st->host_bytes += hb.rd[0] * rec;
+        st->bottom_up_host_bytes += hb.rd[0] * rec;
         // records in HBM, one frontier-bit probe per record, 12 bytes per
         // unvisited vertex, and the visited / frontier bitmaps
         st->hbm_bytes += hb.rd[1] * rec + 4 * (hb.rd[0] + hb.rd[1]) +
@@ -320,6 +353,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     st->ms = (now_ns() - t0) * 1e-6;
     st->expand_ms = expand_ms;
     st->expand_launches = launches;
+    st->bottom_up_ms = bottom_up_ms;
   }
 }
 

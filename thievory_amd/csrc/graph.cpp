@@ -224,6 +224,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   std::vector<uint32_t> rbit(bitmap_words(g.n), 0u), vpart(g.n);
   const uint64_t slab_base = (uint64_t)(uintptr_t)s.slab.p;
   const uint64_t host_base = (uint64_t)(uintptr_t)g.h_edges.dev;
+  s.host_vertices = 0;
   for (uint32_t p = 0; p < s.P; p++) {
     uint64_t pe0 = g.h_off[s.pv[p]];
     for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) {
@@ -233,6 +234,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
         rbit[v >> 5] |= 1u << (v & 31);
       } else {
         eptr[v] = host_base + g.h_off[v] * rec;
+        s.host_vertices += g.h_off[v + 1] != g.h_off[v];
       }
     }
   }

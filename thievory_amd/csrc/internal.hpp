@@ -76,6 +76,7 @@ struct Store {
   uint32_t P = 0;                    // partitions
   uint32_t Pdev = 0;                 // partitions resident in HBM
   uint64_t dev_edges = 0;            // edges resident in HBM
+  uint64_t host_vertices = 0;        // vertices with edges in host memory
   std::vector<uint32_t> pv;          // P+1 first vertex of each partition
   std::vector<uint8_t> pres;         // P residency (1 = HBM)
   DevMem slab;                       // resident records
@@ -162,6 +163,10 @@ struct Graph {
   bool sym = false;           // every edge has its reverse
   bool dir_opt = true;        // direction-optimising BFS when sym
   double do_alpha = 14.0, do_beta = 24.0;
+  // link cost model of a bottom-up level over a split store: each unvisited
+  // vertex with host-resident edges costs one small zero-copy read
+  double do_vertex_ns = 1.5;  // per unvisited host vertex (measured on MI355X)
+  double do_link_gbps = 55.0; // sustained zero-copy read rate
   ~Graph();
 };
 
