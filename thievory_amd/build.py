@@ -35,6 +35,8 @@ LIB_SOURCES = ["graph.cpp", "io.cpp", "abi.cpp", "frontier.hip", "expand.hip",
                "algos.hip", "rmat.hip", "batch.hip", "serve.cpp", "autotune.cpp",
                "bottomup.hip"]
 CLI_SOURCES = ["cli.cpp"]
+# stand-alone host tools (no GPU): format converter and values.bin comparer
+TOOLS = {"thv_convert": "convert.cpp", "thv_compare": "compare.cpp"}
 
 
 def _obj(src):
@@ -87,6 +89,12 @@ def build(force=False, jobs=None, verbose=True):
         subprocess.run(cmd, check=True)
         if verbose:
             print("built", LIB)
+    for exe, src in TOOLS.items():
+        out, path = os.path.join(LIB_DIR, exe), os.path.join(CSRC, src)
+        if force or _stale(out, [path]):
+            subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-o", out, path], check=True)
+            if verbose:
+                print("built", out)
     cli_objs = [_obj(s) for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if cli_objs and (force or _stale(CLI, cli_objs + [LIB])):
         cmd = ["g++", "-o", CLI] + cli_objs + ["-L", LIB_DIR, "-lthv_hip",
