@@ -144,7 +144,21 @@ def serving_bench(T, args):
     return out
 
 
-def bfs_roofline(stats):
+def measured_traffic(workload, kernel):
+    """Memory traffic per launch of `kernel` in `workload` from the committed
+    counter summary profiles/traffic.json: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE per dispatch (KiB), each divided by what the counter reads
+    for a plain copy of known size (thievory_amd/calibrate.py).  None when
+    no such measurement is on file."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return int(e["bytes_per_launch"]) if e else None
+
+
+def bfs_roofline(stats, workload=None):
     """Roofline of the BFS steps.  Kernel view: host-link bytes of each
     traversal kernel over its HIP-event time (top-down expansion levels and
     bottom-up levels are separate kernels).  Mixed views: T_roof = max(HBM
@@ -156,8 +170,8 @@ def bfs_roofline(stats):
     for name, ms_key, launches, host in (
             ("k_expand<BFS>", "expand_ms", lambda s: s.expand_launches,
              lambda s: s.host_bytes - s.bottom_up_host_bytes),
-            ("k_bottom_up<BFS>", "bottom_up_ms", lambda s: s.bottom_up_levels,
-             lambda s: s.bottom_up_host_bytes)):
+            ("bottom-up level (k_expand<BFS_PULL> + k_bottom_up)", "bottom_up_ms",
+             lambda s: s.bottom_up_levels, lambda s: s.bottom_up_host_bytes)):
         t = sum(getattr(s, ms_key) for s in stats)
         n = sum(launches(s) for s in stats)
         b = sum(host(s) for s in stats)
@@ -174,7 +188,8 @@ def bfs_roofline(stats):
     return {
         "bound": "host-link", "kernel": dom, "achieved": d["achieved_GBps"],
         "peak": LINK_PEAK / 1e9, "unit": "GB/s",
-        "frac": round(d["achieved_GBps"] * 1e9 / LINK_PEAK, 4), "traffic": None,
+        "frac": round(d["achieved_GBps"] * 1e9 / LINK_PEAK, 4),
+        "traffic": measured_traffic(workload, dom),
         "avg_launch_ms": d["avg_launch_ms"], "launches": d["launches"],
         "kernels": kern,
         "mixed": {"t_roof_ms": round(t_act, 3), "t_ms": round(med, 3),
@@ -345,14 +360,14 @@ def main():
     trav = sum(s.edges_traversed for s in stats)
     value = trav / wall / 1e9
     ms_per_step = wall * 1e3 / args.steps
-    roof = bfs_roofline(stats)
+    workload = f"bfs-rmat{args.scale}-ef{args.edge_factor}-budget{int(args.budget * 100)}pct"
+    roof = bfs_roofline(stats, workload)
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (seeded R-MAT, generated on the GPU)",
-        "config": {"workload": f"bfs-rmat{args.scale}-ef{args.edge_factor}-budget"
-                               f"{int(args.budget * 100)}pct",
+        "config": {"workload": workload,
                    "scale": args.scale, "edge_factor": args.edge_factor, "vertices": g.n,
                    "edges": g.m, "edge_bytes": g.edge_bytes, "budget_frac": args.budget,
                    "partition_edges": args.partition_edges, "source": args.source,

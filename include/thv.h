@@ -78,6 +78,8 @@ typedef struct {
   double bottom_up_ms;         /* time inside bottom-up kernels (events);
                                   expand_ms counts the expansion kernel only */
   uint64_t bottom_up_host_bytes; /* part of host_bytes read bottom-up       */
+  uint64_t hint_hits;            /* bottom-up BFS: vertices placed by their
+                                    parent hint, without reading their list */
 } thv_stats;
 
 /* ---- graph lifetime --------------------------------------------------- */
@@ -122,11 +124,32 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       (default 14)
  *   THV_OPT_DO_BETA     all edges in HBM: bottom-up -> top-down when the
  *                       frontier holds fewer than n / beta vertices (24)
- *   THV_OPT_DO_VERTEX_NS  split store: a level runs bottom-up when
- *                       (unvisited host vertices x this many ns) is less
- *                       than (frontier host edge bytes / link rate);
- *                       default 1.5 (measured on MI355X)
+ *   THV_OPT_DO_VERTEX_NS  split store: a level runs bottom-up when its
+ *                       estimated link time is less than (frontier host
+ *                       edge bytes / link rate); each unvisited vertex with
+ *                       a long host list costs this many ns (one small
+ *                       zero-copy read); default 3 (measured on MI355X)
  *   THV_OPT_DO_LINK_GBPS  the link rate of that rule, default 55
+ *   THV_OPT_DO_PULL_DEGREE  a bottom-up level scans the whole list of every
+ *                       unvisited vertex of at most this degree with the
+ *                       edge-balanced kernel (coalesced reads); longer
+ *                       lists stop at the first frontier neighbour.
+ *                       Default 16; 0 sends every vertex to the early-exit
+ *                       scan.  The split-store rule then prices a
+ *                       bottom-up level as (bytes of those short lists on
+ *                       the host / link rate) + (longer unvisited host
+ *                       lists x DO_VERTEX_NS).
+ *   THV_OPT_PARENT_HINTS  1 (default): on a split store of a symmetric graph
+ *                       with direction-optimising BFS, n x 4 bytes of the
+ *                       HBM budget hold one parent hint per vertex -- the
+ *                       neighbour of largest degree of every vertex whose
+ *                       list is in host memory.  A bottom-up level first
+ *                       tests the hints; a vertex whose hint is in the
+ *                       frontier joins the next level without its list
+ *                       being read.  0: no hints, the whole budget holds
+ *                       partitions.  Changing it, SYMMETRIC or DIRECTION
+ *                       rebuilds the store with the same budget.
+ *   THV_OPT_HINT_BYTES  (read only) HBM bytes the hints use, 0 if none.
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -134,6 +157,9 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_DO_BETA 4
 #define THV_OPT_DO_VERTEX_NS 5
 #define THV_OPT_DO_LINK_GBPS 6
+#define THV_OPT_DO_PULL_DEGREE 7
+#define THV_OPT_PARENT_HINTS 8
+#define THV_OPT_HINT_BYTES 9
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

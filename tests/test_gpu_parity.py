@@ -141,6 +141,77 @@ def test_bfs_direction_optimizing(rmat16, frac):
     g.set_option("do_alpha", 14.0)
 
 
+@pytest.mark.parametrize("frac", [1.0, 0.25, 0.0])
+@pytest.mark.parametrize("pull_degree", [0, 1, 16, 1024])
+def test_bfs_bottom_up_full_scan_split(rmat16, frac, pull_degree):
+    """Bottom-up levels split between the full-list scan (degree <= D) and
+    the early-exit scan give the same levels for every split point; a slow
+    link rate makes the split-store rule choose bottom-up at this size."""
+    g, og = rmat16
+    _say("case: hybrid bottom-up", frac, pull_degree)
+    g.store_config(int(frac * g.m * g.edge_bytes), 4096)
+    g.set_option("do_pull_degree", pull_degree)
+    g.set_option("do_link_gbps", 0.5)
+    deg = og.degrees()
+    try:
+        for s in (1, 7, 65535):
+            ref = O.bfs(og, s)
+            lv, st = g.bfs(s)
+            assert np.array_equal(lv, ref), s
+            assert st.edges_traversed == int(deg[ref != og.n + 1].sum())
+            if s == 1:
+                assert st.bottom_up_levels > 0
+            if frac == 1.0:
+                assert st.host_bytes == 0
+    finally:
+        g.set_option("do_pull_degree", 16)
+        g.set_option("do_link_gbps", 55)
+    assert g.get_option("do_pull_degree") == 16.0
+
+
+def test_parent_hints(rmat16):
+    """Parent hints take n x 4 bytes of the budget on a split store of a
+    symmetric graph, place vertices without reading their lists, and leave
+    the levels unchanged; switching them off gives the bytes back to the
+    partitions."""
+    g, og = rmat16
+    _say("case: parent hints")
+    budget = int(0.25 * g.m * g.edge_bytes)
+    g.store_config(budget, 4096)
+    assert g.get_option("parent_hints") == 1.0
+    assert g.get_option("hint_bytes") == 4 * g.n
+    dev_with = g.store_info()["device_edges"]
+    assert dev_with * g.edge_bytes <= budget - 4 * g.n
+    g.set_option("do_link_gbps", 0.5)   # bottom-up levels at this size
+    try:
+        for s in (1, 7, 65535):
+            lv, st = g.bfs(s)
+            assert np.array_equal(lv, O.bfs(og, s)), s
+            if s == 1:
+                assert st.bottom_up_levels > 0 and st.hint_hits > 0
+        g.set_option("parent_hints", 0)   # rebuilds the store without hints
+        assert g.get_option("hint_bytes") == 0
+        assert g.store_info()["device_edges"] > dev_with
+        lv, st = g.bfs(1)
+        assert np.array_equal(lv, O.bfs(og, 1)) and st.hint_hits == 0
+        g.set_option("parent_hints", 1)
+        assert g.get_option("hint_bytes") == 4 * g.n
+        g.set_option("direction", 0)      # no bottom-up levels: no hints
+        assert g.get_option("hint_bytes") == 0
+        g.set_option("direction", 1)
+        assert g.get_option("hint_bytes") == 4 * g.n
+        assert np.array_equal(g.bfs(12345)[0], O.bfs(og, 12345))
+    finally:
+        g.set_option("do_link_gbps", 55)
+    with pytest.raises(T.ThvError):
+        g.set_option("hint_bytes", 1)
+    g.store_config(int(0.0 * g.m), 4096)   # budget below 8n bytes: no hints
+    assert g.get_option("hint_bytes") == 0
+    gd = T.Graph.rmat(12, 8, symmetric=False)
+    gd.store_config(int(0.25 * gd.m * 4), 512)
+    assert gd.get_option("hint_bytes") == 0
+
+
 def test_symmetric_hint_on_arrays():
     og = O.rmat(12, 16, seed=11)
     g = T.Graph.from_arrays(og.off, og.adj)

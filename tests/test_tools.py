@@ -106,3 +106,26 @@ def test_compare_rules(tools, tmp_path):
     u.tofile(tmp_path / "u")
     assert _cmp(cmpb, tmp_path / "u", tmp_path / "u", 1, 2).returncode == 0
     assert _cmp(cmpb, tmp_path / "u", tmp_path / "u", 1, 3).returncode == 2  # short file
+
+
+def test_profsum_tables(tmp_path, capsys):
+    """The rocprofv3 CSV summariser keeps template instances apart and
+    divides counters by the calibration factor."""
+    from thievory_amd import profsum
+    tr = tmp_path / "t_kernel_trace.csv"
+    tr.write_text("Kernel_Name,Start_Timestamp,End_Timestamp\n"
+                  "k_expand<0>,0,1000000\nk_expand<0>,0,3000000\nk_expand<5>,10,20\n")
+    profsum.main(["trace", str(tr), "--match", "k_expand"])
+    out = capsys.readouterr().out
+    assert "| `k_expand<0>` | 2 | 4.000 | 2.0000 |" in out
+    assert "`k_expand<5>` | 1 |" in out
+    pm = tmp_path / "p_counter_collection.csv"
+    pm.write_text("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n"
+                  "1,k,FETCH_SIZE,100\n2,k,FETCH_SIZE,300\n")
+    profsum.main(["pmc", str(pm), "--factor", "0.5"])
+    out = capsys.readouterr().out
+    assert "| `k` | FETCH_SIZE | 2 | 400 | 200.0 | 400.0 |" in out
+    st = tmp_path / "s_kernel_stats.csv"
+    st.write_text("Name,Calls,TotalDurationNs,AverageNs\nk,4,8000000,2000000\n")
+    profsum.main(["stats", str(st)])
+    assert "| `k` | 4 | 8.000 | 2.0000 | 100.0% |" in capsys.readouterr().out

@@ -27,7 +27,7 @@ FMT = {"csr": 0, "csc": 1, "wcsr": 2}
 ALGO = {"bfs": 0, "cc": 1, "sssp": 2, "pr": 3}
 PR_MODE = {"push": 0, "pull": 1}
 OPTIONS = {"symmetric": 1, "direction": 2, "do_alpha": 3, "do_beta": 4, "do_vertex_ns": 5,
-           "do_link_gbps": 6}
+           "do_link_gbps": 6, "do_pull_degree": 7, "parent_hints": 8, "hint_bytes": 9}
 ERRORS = {0: "OK", -1: "EINVAL", -2: "EIO", -3: "ENOMEM", -4: "EHIP", -5: "EFORMAT",
           -6: "EBUSY", -7: "ENOTSUP", -8: "ESTATE"}
 
@@ -50,7 +50,7 @@ class Stats(C.Structure):
                 ("edges_examined", C.c_uint64), ("bottom_up_levels", C.c_uint32),
                 ("pad", C.c_uint32), ("topdown_host_bytes", C.c_uint64),
                 ("topdown_hbm_bytes", C.c_uint64), ("bottom_up_ms", C.c_double),
-                ("bottom_up_host_bytes", C.c_uint64)]
+                ("bottom_up_host_bytes", C.c_uint64), ("hint_hits", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}

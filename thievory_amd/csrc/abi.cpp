@@ -75,12 +75,35 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
     NEED(gh);
     Graph *g = reinterpret_cast<Graph *>(gh);
     std::lock_guard<std::mutex> lk(g->mu);
+    // options that decide the parent hints may rebuild the store
+    auto refresh = [&] {
+      if (g->servers.load() > 0 && g->store.configured &&
+          store_wants_hints(*g, g->store.budget) != g->store.hint_reserved)
+        fail(THV_EBUSY, "stop the servers first: this change rebuilds the store");
+    };
     switch (option) {
-      case THV_OPT_SYMMETRIC:
+      case THV_OPT_SYMMETRIC: {
         if (g->csc && value != 0.0) fail(THV_EINVAL, "a CSC graph holds in-edges only");
+        const bool old = g->sym;
         g->sym = value != 0.0;
+        try { refresh(); } catch (...) { g->sym = old; throw; }
+        store_refresh(*g);
         break;
-      case THV_OPT_DIRECTION: g->dir_opt = value != 0.0; break;
+      }
+      case THV_OPT_DIRECTION: {
+        const bool old = g->dir_opt;
+        g->dir_opt = value != 0.0;
+        try { refresh(); } catch (...) { g->dir_opt = old; throw; }
+        store_refresh(*g);
+        break;
+      }
+      case THV_OPT_PARENT_HINTS: {
+        const bool old = g->parent_hints;
+        g->parent_hints = value != 0.0;
+        try { refresh(); } catch (...) { g->parent_hints = old; throw; }
+        store_refresh(*g);
+        break;
+      }
       case THV_OPT_DO_ALPHA:
         NEED(value > 0.0);
         g->do_alpha = value;
@@ -96,6 +119,10 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
       case THV_OPT_DO_LINK_GBPS:
         NEED(value > 0.0);
         g->do_link_gbps = value;
+        break;
+      case THV_OPT_DO_PULL_DEGREE:
+        NEED(value >= 0.0 && value <= 1024.0);
+        g->do_pull_degree = (uint32_t)value;
         break;
       default: fail(THV_EINVAL, "unknown option");
     }
@@ -113,6 +140,9 @@ int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
       case THV_OPT_DO_BETA: *value = g->do_beta; break;
       case THV_OPT_DO_VERTEX_NS: *value = g->do_vertex_ns; break;
       case THV_OPT_DO_LINK_GBPS: *value = g->do_link_gbps; break;
+      case THV_OPT_DO_PULL_DEGREE: *value = g->do_pull_degree; break;
+      case THV_OPT_PARENT_HINTS: *value = g->parent_hints ? 1.0 : 0.0; break;
+      case THV_OPT_HINT_BYTES: *value = (double)g->store.d_hint.bytes; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });

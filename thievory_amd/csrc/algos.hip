@@ -184,7 +184,9 @@ static void value_loop(Graph &g, Algo algo, thv_stats *st) {
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
-    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s);
+    // BFS: the level-L frontier holds the vertices at value L + 1
+    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, 0,
+                   algo == A_BFS ? w.val.as<uint32_t>() : nullptr, L + 1);
     HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
     a.ctr = c;
     a.next = w.front[cur ^ 1].as<uint32_t>();
@@ -229,12 +231,22 @@ static void check_src(Graph &g, uint32_t src) {
 }
 
 // Direction-optimising BFS on a symmetric graph (Beamer et al., SC'12).
-// Each level first counts the frontier (vertices n_f, edges m_f); with
-// m_u the edges of still-unvisited vertices, the level runs bottom-up when
-// m_f > m_u / alpha and returns to top-down once n_f < n / beta and the
-// frontier shrinks.  Top-down levels are the ordinary compaction +
-// expansion; bottom-up levels compact the unvisited set (inverted visited
-// bitmap) and let each vertex search its own list for a frontier parent.
+// Each level first counts the frontier (vertices n_f, edges m_f).  All
+// edges in HBM: with m_u the edges of still-unvisited vertices, the level
+// runs bottom-up when m_f > m_u / alpha and returns to top-down once
+// n_f < n / beta and the frontier shrinks.  Split store: what matters is
+// the host link, so when the frontier's host edges are worth it the
+// parent hints are tested first (every unvisited vertex whose hint is in
+// the frontier joins the next level: it is at level L+1 whichever way the
+// level runs), the rest of the unvisited set is counted, and the cheaper
+// direction in link time is taken (below).
+//
+// A top-down level is the ordinary compaction + expansion.  A bottom-up
+// level compacts the unvisited set (inverted visited bitmap) in two parts:
+// vertices of degree <= D (do_pull_degree) are scanned in full by the
+// edge-balanced kernel (their lists are a few records, read as coalesced
+// wave loads shared by neighbouring queue entries), longer lists go to the
+// early-exit kernel, which stops at the first frontier neighbour.
 // edges_traversed keeps the algorithm-independent count (sum of degrees of
 // reached vertices); edges_examined counts the records actually read.
 static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
@@ -243,107 +255,164 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const uint64_t words = bitmap_words(g.n);
   ExpandArgs a = base_args(g);
   LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
+  LevelCtr *c_long = d_ctr + 2, *c_short = d_ctr + 3;
   const uint64_t rec = (uint64_t)g.rec;
+  const uint32_t D = g.do_pull_degree;
+  const uint32_t *vis = w.vis.as<uint32_t>();
   int cur = 0;
   bool bu = false;
   const bool trace = getenv("THV_TRACE") != nullptr;
   const bool host_share = g.store.host_vertices > 0;
-  uint64_t visited_edges = 0, visited_host = 0, prev_nf = 0;
+  const double link = g.do_link_gbps * 1e9;
+  const bool hints = host_share && g.store.d_hint.p != nullptr;
+  unsigned long long *d_hits = w.aux.as<unsigned long long>() + 4;
+  // per-vertex costs of a bottom-up level over host lists: a full scan of
+  // a short list costs its bytes plus SHORT_NS (a partly shared request),
+  // an early-exit scan of a long list do_vertex_ns (measured on MI355X)
+  constexpr double SHORT_NS = 0.6;
+  uint64_t visited_edges = 0, prev_nf = 0, seen = 0;
   double expand_ms = 0, bottom_up_ms = 0;
   uint32_t launches = 0;
+  if (hints) HIPC(hipMemsetAsync(d_hits, 0, 8, s));
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
-    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_COUNT_ONLY);
+    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_COUNT_ONLY,
+                   w.val.as<uint32_t>(), L + 1);
     HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const LevelCtr h = w.h_ctr[L & 1];
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
     if (nf == 0) break;
+    seen += nf;
     visited_edges += mf;
-    visited_host += h.qn[0];
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
+    // probed: the unvisited set was counted, so the tile tables now hold
+    // the short-list count (long first, short last) and the frontier's
+    // write pass needs a fresh count
+    bool probed = false, hinted = false;
+    double td_cost = 0, bu_cost = 0;
+    HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
     if (host_share) {
-      // split store: a top-down level streams the frontier's host edges
-      // over the link; a bottom-up level pays one small zero-copy read per
-      // unvisited host vertex (request-rate bound)
-      const uint64_t unv_host = g.store.host_vertices - std::min(g.store.host_vertices, visited_host);
-      const double td = (double)h.qw[0] * rec / (g.do_link_gbps * 1e9);
-      const double bu_cost = (double)unv_host * g.do_vertex_ns * 1e-9;
-      bu = bu_cost < td;
+      // top-down streams the frontier's host records over the link; the
+      // hybrid bottom-up level streams the short host lists in full and
+      // pays one small zero-copy read per long unvisited host list.  The
+      // two count passes (about 6 HBM bytes per vertex and a launch +
+      // readback each) run only when the frontier's host records cost more.
+      td_cost = (double)h.qw[0] * rec / link;
+      const double probe_cost = 2.0 * ((double)g.n * 6.0 / 5e12 + 15e-6);
+      bu = false;
+      if (td_cost > probe_cost) {
+        if (hints) {
+          launch_hint_pass(g, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(),
+                           w.front[cur ^ 1].as<uint32_t>(), d_hits, s);
+          hinted = true;
+        }
+        launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_COUNT_ONLY, nullptr, 0,
+                       D + 1, 0xFFFFFFFFu);
+        if (D > 0)
+          launch_compact(g, w.qs, vis, c_short, s, CMP_INVERT | CMP_COUNT_ONLY, nullptr, 0, 1, D);
+        HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+        if (trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_hits, 8, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        const uint64_t short_host = D > 0 ? w.h_ctr[3].qw[0] : 0;
+        const uint64_t short_host_n = D > 0 ? w.h_ctr[3].qn[0] : 0;
+        bu_cost = (double)short_host * rec / link + (double)short_host_n * SHORT_NS * 1e-9 +
+                  (double)w.h_ctr[2].qn[0] * g.do_vertex_ns * 1e-9;
+        bu = bu_cost < td_cost;
+        probed = true;
+      }
     } else {
       // everything in HBM: the edge-ratio rule of Beamer et al.
       if (!bu && (double)mf > (double)mu / g.do_alpha) bu = true;
       else if (bu && (double)nf < (double)g.n / g.do_beta && nf < prev_nf) bu = false;
     }
     prev_nf = nf;
-    HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
+    a.next = w.front[cur ^ 1].as<uint32_t>();
+    a.front = w.front[cur].as<uint32_t>();
     if (!bu) {
-      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_WRITE_ONLY);
+      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, probed ? 0 : CMP_WRITE_ONLY);
       a.ctr = c;
-      a.next = w.front[cur ^ 1].as<uint32_t>();
       a.level_value = L + 2;
       HIPC(hipEventRecord(w.ev[0], s));
       launch_expand(g, A_BFS, a, s);
       HIPC(hipEventRecord(w.ev[1], s));
     } else {
-      LevelCtr *cb = d_ctr + 2;
-      launch_compact(g, w.qs, w.vis.as<uint32_t>(), cb, s, CMP_INVERT | CMP_NO_CHUNKS);
+      HIPC(hipEventRecord(w.ev[0], s));
+      if (D > 0) {
+        launch_compact(g, w.qs, vis, c_short, s, CMP_INVERT | (probed ? CMP_WRITE_ONLY : 0),
+                       nullptr, 0, 1, D);
+        ExpandArgs p = a;
+        p.ctr = c_short;
+        launch_expand(g, A_BFS_PULL, p, s);
+      }
+      // vertices found by the full scan are visited now but are short, so
+      // the long set is the same as when it was counted
+      launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_NO_CHUNKS, nullptr, 0, D + 1,
+                     0xFFFFFFFFu);
       BottomUpArgs b;
       b.qv[0] = w.qs.qv[0].as<uint32_t>();
       b.qv[1] = w.qs.qv[1].as<uint32_t>();
-      b.ctr = cb;
+      b.ctr = c_long;
       b.eptr = g.store.d_eptr.as<uint64_t>();
       b.deg = g.d_deg.as<uint32_t>();
       b.front = w.front[cur].as<uint32_t>();
       b.vis = w.vis.as<uint32_t>();
       b.next = w.front[cur ^ 1].as<uint32_t>();
       b.val = w.val.as<uint32_t>();
-      b.rd_ctr = cb;
+      b.rd_ctr = c_long;
       b.level_value = L + 2;
-      HIPC(hipEventRecord(w.ev[0], s));
       launch_bottom_up(g, b, s);
       HIPC(hipEventRecord(w.ev[1], s));
-      HIPC(hipMemcpyAsync(&w.h_ctr[2], cb, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+      HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     }
     HIPC(hipStreamSynchronize(s));
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
+    memset(&hs, 0, sizeof hs);
+    memset(&hb, 0, sizeof hb);
     if (bu) {
+      hb = w.h_ctr[2];
+      if (D > 0) hs = w.h_ctr[3];
       bottom_up_ms += ms;
     } else {
       expand_ms += ms;
       launches++;
     }
     if (trace) {
-      const LevelCtr &hb = w.h_ctr[2];
       fprintf(stderr,
-              "[thv] bfs L=%u %s nf=%llu mf=%llu front_host_edges=%llu unvisited=%llu "
-              "read_host=%llu read_hbm=%llu kernel_ms=%.3f\n",
+              "[thv] bfs L=%u %s nf=%llu mf=%llu front_host_edges=%llu est_td_ms=%.3f "
+              "est_bu_ms=%.3f hint_hits_total=%llu short=%llu short_host_edges=%llu long=%llu "
+              "long_host=%llu read_host=%llu read_hbm=%llu kernel_ms=%.3f\n",
               L, bu ? "bottom-up" : "top-down", (unsigned long long)nf,
-              (unsigned long long)mf, h.qw[0], bu ? hb.qn[0] + hb.qn[1] : 0ull,
-              bu ? hb.rd[0] : h.qw[0], bu ? hb.rd[1] : h.qw[1], ms);
+              (unsigned long long)mf, h.qw[0], td_cost * 1e3, bu_cost * 1e3,
+              hinted ? w.h_aux[1] : 0ull,
+              hs.qn[0] + hs.qn[1], hs.qw[0], hb.qn[0] + hb.qn[1], hb.qn[0],
+              bu ? hs.qw[0] + hb.rd[0] : h.qw[0], bu ? hs.qw[1] + hb.rd[1] : h.qw[1], ms);
     }
     if (st) {
       st->edges_traversed += mf;
       st->iterations++;
       st->topdown_host_bytes += h.qw[0] * rec;
       st->topdown_hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
+      if (hinted)  // hint words and frontier probes of the unvisited vertices
+        st->hbm_bytes += 8 * (g.n - std::min<uint64_t>(g.n, seen)) + 8 * words;
       if (!bu) {
         st->edges_examined += mf;
         st->host_bytes += h.qw[0] * rec;
         st->hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
         st->host_partitions_touched += h.touched[0];
       } else {
-        const LevelCtr &hb = w.h_ctr[2];
-        st->edges_examined += hb.rd[0] + hb.rd[1];
-        st->host_bytes += hb.rd[0] * rec;
-        st->bottom_up_host_bytes += hb.rd[0] * rec;
+        const uint64_t rh = hs.qw[0] + hb.rd[0], rd = hs.qw[1] + hb.rd[1];
+        st->edges_examined += rh + rd;
+        st->host_bytes += rh * rec;
+        st->bottom_up_host_bytes += rh * rec;
         // records in HBM, one frontier-bit probe per record, 12 bytes per
         // unvisited vertex, and the visited / frontier bitmaps
-        st->hbm_bytes += hb.rd[1] * rec + 4 * (hb.rd[0] + hb.rd[1]) +
-                         12 * (hb.qn[0] + hb.qn[1]) + 12 * words;
-        st->host_partitions_touched += hb.touched[0];
+        st->hbm_bytes += rd * rec + 4 * (rh + rd) +
+                         12 * (hs.qn[0] + hs.qn[1] + hb.qn[0] + hb.qn[1]) + 12 * words;
+        st->host_partitions_touched += hb.touched[0] + hs.touched[0];
         st->bottom_up_levels++;
       }
     }
@@ -354,6 +423,10 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     st->expand_ms = expand_ms;
     st->expand_launches = launches;
     st->bottom_up_ms = bottom_up_ms;
+    if (hints) {
+      HIPC(hipMemcpy(w.h_aux, d_hits, 8, hipMemcpyDeviceToHost));
+      st->hint_hits = w.h_aux[0];
+    }
   }
 }
 

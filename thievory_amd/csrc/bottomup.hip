@@ -13,6 +13,8 @@
 // ballot ends the scan at the first hit.  Host and HBM queue entries are
 // interleaved like the expansion chunks.  Records read are counted per
 // residency for the byte accounting.
+#include <algorithm>
+
 #include "internal.hpp"
 #include "scan.hpp"
 
@@ -55,8 +57,7 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
     }
     if (gl == 0) {
       rd[q] += found ? min(d, k + G) : d;
-      if (found) {
-        a.val[v] = a.level_value;
+      if (found) {  // level written by the next compaction, like top-down
         atomicOr(&a.next[v >> 5], 1u << (v & 31));
         atomicOr(&a.vis[v >> 5], 1u << (v & 31));
       }
@@ -69,12 +70,119 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
   }
 }
 
+// Parent hints: a group of 16 lanes per vertex whose list is in host
+// memory scans the list (zero-copy) for the neighbour of largest degree,
+// ties to the smallest id.  BFS reaches high-degree vertices early, so in
+// a bottom-up level this one HBM word usually names a frontier neighbour.
+// Built once per store configuration, outside any traversal.
+template <int REC>
+__global__ void __launch_bounds__(BBS) k_hint_build(const uint64_t *eptr, const uint32_t *deg,
+                                                   const uint32_t *rbit, uint64_t n,
+                                                   uint32_t *hint) {
+  const uint32_t gl = threadIdx.x & (G - 1);
+  const unsigned long long ngroups = (unsigned long long)gridDim.x * BBS / G;
+  for (unsigned long long v = (blockIdx.x * (unsigned long long)BBS + threadIdx.x) / G; v < n;
+       v += ngroups) {
+    // v, d and host are the same on the 16 lanes of a group, so the
+    // group's lanes reach the shuffles together
+    const uint32_t d = deg[v];
+    const bool host = d != 0 && !((rbit[v >> 5] >> (v & 31)) & 1u);
+    unsigned long long best = 0;  // (degree << 32) | (~id): max = largest degree, smallest id
+    if (host) {
+      const unsigned long long base = eptr[v];
+      for (uint32_t k = gl; k < d; k += G) {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (unsigned long long)k * REC);
+        const unsigned long long key = ((unsigned long long)deg[w] << 32) | (0xFFFFFFFFu - w);
+        best = best > key ? best : key;
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(best, o, 64);
+        best = best > other ? best : other;
+      }
+    }
+    if (gl == 0) hint[v] = host ? 0xFFFFFFFFu - (uint32_t)best : NO_HINT;
+  }
+}
+
+// One wave per 1024 vertices (32 bitmap words), in three batches of loads:
+// lanes 0-31 read the 32 visited words, every lane then reads the hints of
+// its 16 unvisited vertices, then their frontier bits.  Each word's hits are
+// gathered in the lane that read it, which writes the word once (no
+// atomics: no other kernel runs on these bitmaps meanwhile).
+constexpr uint32_t HSUB = 1024, HROUNDS = HSUB / 64;
+
+__global__ void __launch_bounds__(BBS) k_hint_pass(const uint32_t *hint, const uint32_t *front,
+                                                  uint32_t *vis, uint32_t *next, uint64_t n,
+                                                  unsigned long long *hits) {
+  const uint32_t lane = threadIdx.x & 63;
+  const unsigned long long nwaves = (unsigned long long)gridDim.x * BBS / 64;
+  unsigned long long cnt = 0;
+  for (unsigned long long wv = (blockIdx.x * (unsigned long long)BBS + threadIdx.x) / 64;
+       wv * HSUB < n; wv += nwaves) {
+    const uint64_t base = wv * HSUB, w0 = base >> 5;
+    const bool owner = lane < 32 && ((w0 + lane) << 5) < n;
+    const uint32_t vw = owner ? vis[w0 + lane] : 0xFFFFFFFFu;
+    uint32_t h[HROUNDS];
+#pragma unroll
+    for (uint32_t r = 0; r < HROUNDS; r++) {
+      const uint32_t word = (uint32_t)__shfl((int)vw, (int)(2 * r + (lane >> 5)), 64);
+      const uint64_t v = base + r * 64 + lane;
+      h[r] = (v < n && !((word >> (lane & 31)) & 1u)) ? hint[v] : NO_HINT;
+    }
+    uint32_t f = 0;  // bit r: the hint of round r's vertex is in the frontier
+#pragma unroll
+    for (uint32_t r = 0; r < HROUNDS; r++)
+      if (h[r] != NO_HINT && ((front[h[r] >> 5] >> (h[r] & 31)) & 1u)) f |= 1u << r;
+    uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
+#pragma unroll
+    for (uint32_t r = 0; r < HROUNDS; r++) {
+      const unsigned long long b = __ballot((f >> r) & 1u);
+      if (lane == 2 * r) m = (uint32_t)b;
+      if (lane == 2 * r + 1) m = (uint32_t)(b >> 32);
+    }
+    if (owner && m) {
+      vis[w0 + lane] = vw | m;
+      next[w0 + lane] |= m;
+      cnt += __popc(m);
+    }
+  }
+  cnt = wave_sum(cnt);
+  if (lane == 0 && cnt) atomicAdd(hits, cnt);
+}
+
 }  // namespace
 
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
   const int grid = 8 * g.cus;
   if (g.rec == 8) hipLaunchKernelGGL(k_bottom_up<8>, dim3(grid), dim3(BBS), 0, s, a);
   else hipLaunchKernelGGL(k_bottom_up<4>, dim3(grid), dim3(BBS), 0, s, a);
+  HIPC(hipGetLastError());
+}
+
+void build_parent_hints(Graph &g) {
+  Store &st = g.store;
+  st.d_hint.alloc(g.n * 4);
+  const int grid = 8 * g.cus;
+  const uint64_t *eptr = st.d_eptr.as<uint64_t>();
+  const uint32_t *deg = g.d_deg.as<uint32_t>(), *rbit = st.d_rbit.as<uint32_t>();
+  uint32_t *hint = st.d_hint.as<uint32_t>();
+  if (g.rec == 8)
+    hipLaunchKernelGGL(k_hint_build<8>, dim3(grid), dim3(BBS), 0, g.stream, eptr, deg, rbit, g.n,
+                       hint);
+  else
+    hipLaunchKernelGGL(k_hint_build<4>, dim3(grid), dim3(BBS), 0, g.stream, eptr, deg, rbit, g.n,
+                       hint);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(g.stream));
+}
+
+void launch_hint_pass(Graph &g, const uint32_t *front, uint32_t *vis, uint32_t *next,
+                      unsigned long long *hits, hipStream_t s) {
+  const uint64_t waves = div_up(g.n, HSUB);
+  const int grid = (int)std::min<uint64_t>(div_up(waves, BBS / 64), 16ull * g.cus);
+  hipLaunchKernelGGL(k_hint_pass, dim3(grid ? grid : 1), dim3(BBS), 0, s,
+                     g.store.d_hint.as<uint32_t>(), front, vis, next, g.n, hits);
   HIPC(hipGetLastError());
 }
 

@@ -29,6 +29,9 @@
 //   SSSP : atomicMin(val[w], val[v] + weight) (sssp_kernels.cu:124-141)
 //   PR   : push atomicAdd(res[w], delta[v]) (pr_kernels.cu:539-542);
 //          pull sum[v] += contrib[u] over in-edges (pr_kernels.cu:45-51)
+//   BFS pull (bottom-up level on a symmetric graph, queue = unvisited
+//          vertices): an entry whose list holds a frontier vertex joins the
+//          next frontier; same levels as the top-down rule
 #include "chunk.hpp"
 
 namespace thv {
@@ -44,6 +47,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   __shared__ unsigned long long s_base[C + 1];
   __shared__ uint32_t s_u[C + 1];  // CC/SSSP: source value; PR pull: vertex id
   __shared__ float s_f[C + 1];     // PR push: delta; PR pull: partial sum
+  uint32_t *s_hit = reinterpret_cast<uint32_t *>(s_f);  // BFS pull: parent found
 
   const ChunkSrc cs = chunk_src(a.ctr, C);
   const uint32_t tid = threadIdx.x;
@@ -70,6 +74,10 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
         if (ALGO == A_PR_PULL) {
           s_u[t] = v;
           s_f[t] = 0.0f;
+        }
+        if (ALGO == A_BFS_PULL) {
+          s_u[t] = v;
+          s_hit[t] = 0;
         }
       }
     }
@@ -105,11 +113,10 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       const uint32_t wd = w >> 5, bit = 1u << (w & 31);
       if (ALGO == A_BFS) {
         if (!(a.vis[wd] & bit)) {
+          // the winner only marks the next frontier; the next level's
+          // compaction writes the level of every frontier vertex
           const uint32_t old = atomicOr(&a.vis[wd], bit);
-          if (!(old & bit)) {
-            a.val[w] = a.level_value;
-            atomicOr(&a.next[wd], bit);
-          }
+          if (!(old & bit)) atomicOr(&a.next[wd], bit);
         }
       } else if (ALGO == A_CC || ALGO == A_SSSP) {
         const unsigned long long nv64 =
@@ -119,6 +126,8 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
           const uint32_t old = atomicMin(&a.val[w], nv);
           if (nv < old && !(a.next[wd] & bit)) atomicOr(&a.next[wd], bit);
         }
+      } else if (ALGO == A_BFS_PULL) {
+        if (a.front[wd] & bit) s_hit[ent[j]] = 1;
       } else if (ALGO == A_PR_PUSH) {
         unsafeAtomicAdd(&a.pr_dst[w], s_f[ent[j]]);
       } else {  // A_PR_PULL: w is an in-neighbour of the entry's vertex
@@ -129,6 +138,18 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       __syncthreads();
       for (uint32_t t = tid; t < ci.nent; t += EBS) {
         if (chunk_rel(qe[ci.i0 + t], ci.e0) < C) unsafeAtomicAdd(&a.pr_dst[s_u[t]], s_f[t]);
+      }
+    }
+    if (ALGO == A_BFS_PULL) {
+      // a list split over two chunks may be found twice: both ORs are
+      // idempotent and the next compaction writes the level
+      __syncthreads();
+      for (uint32_t t = tid; t < ci.nent; t += EBS) {
+        if (chunk_rel(qe[ci.i0 + t], ci.e0) < C && s_hit[t]) {
+          const uint32_t v = s_u[t];
+          atomicOr(&a.vis[v >> 5], 1u << (v & 31));
+          atomicOr(&a.next[v >> 5], 1u << (v & 31));
+        }
       }
     }
     __syncthreads();
@@ -166,6 +187,9 @@ void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
       break;
     case A_PR_PULL:
       r8 ? launch_t<A_PR_PULL, 8>(g, args, s) : launch_t<A_PR_PULL, 4>(g, args, s);
+      break;
+    case A_BFS_PULL:
+      r8 ? launch_t<A_BFS_PULL, 8>(g, args, s) : launch_t<A_BFS_PULL, 4>(g, args, s);
       break;
   }
   HIPC(hipGetLastError());

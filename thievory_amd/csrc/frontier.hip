@@ -16,8 +16,9 @@
 // A host partition with no active vertex produces no entry, so its edges are
 // never read (the per-partition activity filter).  Queues come out sorted by
 // vertex id, which makes the zero-copy reads of a dense host frontier
-// sequential.  64 vertices whose frontier bits are all clear cost one
-// ballot.
+// sequential.  A wave reads its sub-tile's bitmap words with one load and
+// the degrees of all its selected vertices with one batch of loads, so an
+// empty sub-tile costs one load and a few lane shuffles.
 #include "internal.hpp"
 #include "scan.hpp"
 
@@ -44,38 +45,84 @@ struct CompactArgs {
   uint32_t C;
   uint32_t invert;     // select vertices whose bit is clear
   uint32_t no_chunks;  // skip the chunk-start table
+  uint32_t *val;       // BFS: frontier vertices' level, written by k_tcount
+  uint32_t val_value;
+  uint32_t dmin, dmax; // only vertices with dmin <= degree <= dmax (dmin >= 1)
 };
 
-// Lane's vertex: active (frontier bit, degree > 0), residency queue, degree.
-__device__ __forceinline__ void vertex_state(const CompactArgs &a, uint64_t v, bool &act,
-                                             int &q, uint32_t &d) {
-  act = false;
-  q = Q_HOST;
-  d = 0;
-  if (v < a.n && (((a.front[v >> 5] >> (v & 31)) & 1u) ^ a.invert)) {
-    d = a.deg[v];
-    act = d != 0;
-    if (act && ((a.rbit[v >> 5] >> (v & 31)) & 1u)) q = Q_DEV;
+// One wave's 1024-vertex sub-tile, read in two dependent memory round
+// trips instead of one per 64 vertices: lanes 0-31 load the sub-tile's 32
+// frontier and residency words together, every round takes its word by a
+// lane shuffle, then the degrees of all selected vertices of all rounds are
+// loaded together.  Round r of lane l is vertex base + 64 r + l.
+struct SubTile {
+  uint32_t act;        // bit r: vertex of round r is selected and its degree in range
+  uint32_t dev;        // bit r: its edges are in HBM
+  uint32_t raw;        // bit r: its frontier bit is set (before any inversion)
+  uint32_t d[ROUNDS];  // degree of round r's vertex when selected, else 0
+};
+
+__device__ __forceinline__ void load_subtile(const CompactArgs &a, uint64_t base, uint32_t lane,
+                                             SubTile &t) {
+  const uint64_t w0 = base >> 5;
+  uint32_t fw = 0, rw = 0;
+  if (lane < 32 && ((w0 + lane) << 5) < a.n) {
+    fw = a.front[w0 + lane];
+    rw = a.rbit[w0 + lane];
   }
+  t.act = t.dev = t.raw = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < ROUNDS; r++) {
+    const int src = (int)(2 * r + (lane >> 5));
+    const uint32_t f = (uint32_t)__shfl((int)fw, src, 64);
+    const uint32_t rb = (uint32_t)__shfl((int)rw, src, 64);
+    const uint32_t bit = 1u << (lane & 31);
+    const bool in = base + r * 64 + lane < a.n;
+    const bool fb = (f & bit) != 0;
+    if (in && (fb != (a.invert != 0))) t.act |= 1u << r;
+    if (in && fb) t.raw |= 1u << r;
+    if (rb & bit) t.dev |= 1u << r;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < ROUNDS; r++)
+    t.d[r] = ((t.act >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
+#pragma unroll
+  for (uint32_t r = 0; r < ROUNDS; r++) {
+    if (t.d[r] < a.dmin || t.d[r] > a.dmax) {
+      t.act &= ~(1u << r);
+      t.d[r] = 0;
+    }
+  }
+  t.dev &= t.act;
+}
+
+// Per-lane totals of a sub-tile: entries and edges per queue.
+__device__ __forceinline__ void subtile_sums(const SubTile &t, uint32_t c[2],
+                                             unsigned long long e[2]) {
+  c[Q_DEV] = __popc(t.dev);
+  c[Q_HOST] = __popc(t.act) - c[Q_DEV];
+  e[0] = e[1] = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < ROUNDS; r++) e[(t.dev >> r) & 1u] += t.d[r];
 }
 
 __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
   __shared__ unsigned long long s[CBS / 64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    const uint64_t base = (uint64_t)t * TILE + (uint64_t)wv * SUB;
-    uint32_t c[2] = {0, 0};
-    unsigned long long e[2] = {0, 0};
-    for (uint32_t r = 0; r < ROUNDS; r++) {
-      bool act;
-      int q;
-      uint32_t d;
-      vertex_state(a, base + r * 64 + lane, act, q, d);
-      if (act) {
-        c[q]++;
-        e[q] += d;
-      }
+  for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB;
+    SubTile t;
+    load_subtile(a, base, lane, t);
+    // BFS: every frontier vertex (degree 0 included) takes the level here,
+    // in vertex order, instead of by a scattered store when it was found
+    if (a.val) {
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++)
+        if ((t.raw >> r) & 1u) a.val[base + r * 64 + lane] = a.val_value;
     }
+    uint32_t c[2];
+    unsigned long long e[2];
+    subtile_sums(t, c, e);
     const unsigned long long v0 = wave_sum((unsigned long long)c[0]);
     const unsigned long long v1 = wave_sum((unsigned long long)c[1]);
     const unsigned long long v2 = wave_sum(e[0]);
@@ -90,45 +137,73 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
     if (threadIdx.x < 4) {
       unsigned long long x = 0;
       for (int w = 0; w < CBS / 64; w++) x += s[w][threadIdx.x];
-      if (threadIdx.x < 2) a.tcnt[2 * t + threadIdx.x] = (uint32_t)x;
-      else a.tdeg[2 * t + threadIdx.x - 2] = x;
+      if (threadIdx.x < 2) a.tcnt[2 * tile + threadIdx.x] = (uint32_t)x;
+      else a.tdeg[2 * tile + threadIdx.x - 2] = x;
     }
     __syncthreads();
   }
 }
 
 constexpr int SBS = 1024;
+constexpr int SNW = SBS / 64;
 
+// Exclusive tile bases.  Wave w owns the contiguous tile segment
+// [w * per, (w + 1) * per): a first sweep sums it (64 tiles per coalesced
+// load), the waves' sums are scanned in LDS, and a second sweep writes the
+// bases with a wave scan per 64 tiles.
 __global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
-  __shared__ unsigned long long sh[4][SBS / 64 + 1];
-  unsigned long long carry[4] = {0, 0, 0, 0};
-  for (uint32_t b = 0; b < a.ntiles; b += SBS) {
-    const uint32_t t = b + threadIdx.x;
-    unsigned long long x[4] = {0, 0, 0, 0};
-    if (t < a.ntiles) {
-      x[0] = a.tcnt[2 * t];
-      x[1] = a.tcnt[2 * t + 1];
-      x[2] = a.tdeg[2 * t];
-      x[3] = a.tdeg[2 * t + 1];
-    }
-    unsigned long long ex[4], tot[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) ex[i] = block_excl_scan<SBS>(x[i], sh[i], &tot[i]);
-    if (t < a.ntiles) {
-      a.tbc[2 * t] = (uint32_t)(carry[0] + ex[0]);
-      a.tbc[2 * t + 1] = (uint32_t)(carry[1] + ex[1]);
-      a.tbe[2 * t] = carry[2] + ex[2];
-      a.tbe[2 * t + 1] = carry[3] + ex[3];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) carry[i] += tot[i];
-    __syncthreads();
+  __shared__ unsigned long long sh[SNW][4];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t per = (a.ntiles + SNW - 1) / SNW;
+  const uint32_t s0 = min(a.ntiles, wv * per), s1 = min(a.ntiles, s0 + per);
+  const uint32_t *__restrict__ tcnt = a.tcnt;
+  const unsigned long long *__restrict__ tdeg = a.tdeg;
+  unsigned long long x[4] = {0, 0, 0, 0};
+#pragma unroll 4
+  for (uint32_t t = s0 + lane; t < s1; t += 64) {
+    x[0] += tcnt[2 * t];
+    x[1] += tcnt[2 * t + 1];
+    x[2] += tdeg[2 * t];
+    x[3] += tdeg[2 * t + 1];
   }
-  if (threadIdx.x == 0) {
-    a.ctr->qn[0] = carry[0];
-    a.ctr->qn[1] = carry[1];
-    a.ctr->qw[0] = carry[2];
-    a.ctr->qw[1] = carry[3];
+#pragma unroll
+  for (int i = 0; i < 4; i++) x[i] = wave_sum(x[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) sh[wv][i] = x[i];
+  }
+  __syncthreads();
+  unsigned long long run[4] = {0, 0, 0, 0};
+  for (uint32_t w = 0; w < wv; w++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) run[i] += sh[w][i];
+  }
+  for (uint32_t t0 = s0; t0 < s1; t0 += 64) {
+    const uint32_t t = t0 + lane;
+    const bool ok = t < s1;
+    unsigned long long v[4];
+    v[0] = ok ? tcnt[2 * t] : 0u;
+    v[1] = ok ? tcnt[2 * t + 1] : 0u;
+    v[2] = ok ? tdeg[2 * t] : 0ull;
+    v[3] = ok ? tdeg[2 * t + 1] : 0ull;
+    unsigned long long inc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) inc[i] = wave_incl_scan(v[i]);
+    if (ok) {
+      a.tbc[2 * t] = (uint32_t)(run[0] + inc[0] - v[0]);
+      a.tbc[2 * t + 1] = (uint32_t)(run[1] + inc[1] - v[1]);
+      a.tbe[2 * t] = run[2] + inc[2] - v[2];
+      a.tbe[2 * t + 1] = run[3] + inc[3] - v[3];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) run[i] += __shfl(inc[i], 63, 64);
+  }
+  // the last wave's running sums end at the grand totals
+  if (wv == SNW - 1 && lane == 0) {
+    a.ctr->qn[0] = run[0];
+    a.ctr->qn[1] = run[1];
+    a.ctr->qw[0] = run[2];
+    a.ctr->qw[1] = run[3];
     a.ctr->touched[0] = 0;
     a.ctr->touched[1] = 0;
     a.ctr->rd[0] = 0;
@@ -140,22 +215,20 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
   __shared__ unsigned long long s[CBS / 64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = lanemask_lt();
-  for (uint32_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    if (a.tcnt[2 * t] == 0 && a.tcnt[2 * t + 1] == 0) continue;
-    const uint64_t base = (uint64_t)t * TILE + (uint64_t)wv * SUB;
-    // pass 1: this wave's share, to place it after the waves before it
-    uint32_t c[2] = {0, 0};
-    unsigned long long e[2] = {0, 0};
-    for (uint32_t r = 0; r < ROUNDS; r++) {
-      bool act;
-      int q;
-      uint32_t d;
-      vertex_state(a, base + r * 64 + lane, act, q, d);
-      if (act) {
-        c[q]++;
-        e[q] += d;
-      }
-    }
+  for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    if (a.tcnt[2 * tile] == 0 && a.tcnt[2 * tile + 1] == 0) continue;
+    const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB;
+    SubTile t;
+    load_subtile(a, base, lane, t);
+    // partition ids of the selected vertices, loaded together
+    uint32_t part[ROUNDS];
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++)
+      part[r] = ((t.act >> r) & 1u) ? a.vpart[base + r * 64 + lane] : 0xFFFFFFFFu;
+    // this wave's share, to place it after the waves before it
+    uint32_t c[2];
+    unsigned long long e[2];
+    subtile_sums(t, c, e);
     const unsigned long long w0 = wave_sum((unsigned long long)c[0]);
     const unsigned long long w1 = wave_sum((unsigned long long)c[1]);
     const unsigned long long w2 = wave_sum(e[0]);
@@ -167,21 +240,21 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
       s[wv][3] = w3;
     }
     __syncthreads();
-    unsigned long long cb[2] = {a.tbc[2 * t], a.tbc[2 * t + 1]};
-    unsigned long long eb[2] = {a.tbe[2 * t], a.tbe[2 * t + 1]};
+    unsigned long long cb[2] = {a.tbc[2 * tile], a.tbc[2 * tile + 1]};
+    unsigned long long eb[2] = {a.tbe[2 * tile], a.tbe[2 * tile + 1]};
     for (uint32_t w = 0; w < wv; w++) {
       cb[0] += s[w][0];
       cb[1] += s[w][1];
       eb[0] += s[w][2];
       eb[1] += s[w][3];
     }
-    // pass 2: ordered entries
+    // ordered entries
+#pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
       const uint64_t v = base + r * 64 + lane;
-      bool act;
-      int q;
-      uint32_t d;
-      vertex_state(a, v, act, q, d);
+      const bool act = (t.act >> r) & 1u;
+      const int q = ((t.dev >> r) & 1u) ? Q_DEV : Q_HOST;
+      const uint32_t d = t.d[r];
       const unsigned long long bh = __ballot(act && q == Q_HOST);
       const unsigned long long bd = __ballot(act && q == Q_DEV);
       if (!(bh | bd)) continue;
@@ -189,8 +262,7 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
       const unsigned long long xd = (act && q == Q_DEV) ? d : 0ull;
       const unsigned long long ih = wave_incl_scan(xh), id = wave_incl_scan(xd);
       if (act) {
-        const unsigned long long idx =
-            cb[q] + __popcll((q == Q_HOST ? bh : bd) & lt);
+        const unsigned long long idx = cb[q] + __popcll((q == Q_HOST ? bh : bd) & lt);
         const unsigned long long e0 = eb[q] + (q == Q_HOST ? ih - xh : id - xd);
         a.qv[q][idx] = (uint32_t)v;
         a.qe[q][idx] = e0;
@@ -200,11 +272,14 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
             qc[ch] = (uint32_t)idx;
         }
       }
-      // partitions touched: the first lane of every run of equal partition
-      // ids stamps it; a stamp that changes counts the partition once
-      const uint32_t p = act ? a.vpart[v] : 0xFFFFFFFFu;
-      const uint32_t pp = __shfl_up(p, 1, 64);
-      if (act && (lane == 0 || pp != p)) {
+      // partitions touched: among the active lanes, the first of every run
+      // of equal partition ids stamps it; a stamp that changes counts the
+      // partition once
+      const uint32_t p = part[r];
+      const unsigned long long below = (bh | bd) & lt;
+      const int prev = below ? 63 - __clzll((long long)below) : (int)lane;
+      const uint32_t pp = (uint32_t)__shfl((int)p, prev, 64);
+      if (act && (!below || pp != p)) {
         if (atomicExch(&a.ptouch[p], a.stamp) != a.stamp) atomicAdd(&a.ctr->touched[q], 1u);
       }
       cb[0] += __popcll(bh);
@@ -219,7 +294,9 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
 }  // namespace
 
 void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
-                    hipStream_t s, int mode) {
+                    hipStream_t s, int mode, uint32_t *val, uint32_t val_value, uint32_t dmin,
+                    uint32_t dmax) {
+  if (dmin == 0) dmin = 1;  // degree-0 vertices never have work
   Store &st = g.store;
   CompactArgs a;
   memset(&a, 0, sizeof a);
@@ -248,6 +325,10 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   a.C = w.C;
   a.invert = (mode & CMP_INVERT) ? 1u : 0u;
   a.no_chunks = (mode & CMP_NO_CHUNKS) ? 1u : 0u;
+  a.val = (mode & (CMP_INVERT | CMP_WRITE_ONLY)) ? nullptr : val;
+  a.val_value = val_value;
+  a.dmin = dmin;
+  a.dmax = dmax;
   const uint32_t grid = a.ntiles < 65535u ? a.ntiles : 65535u;
   // WRITE_ONLY continues a COUNT_ONLY call on the same bitmap and counters:
   // the tile bases and totals are still in place

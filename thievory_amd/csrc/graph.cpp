@@ -151,6 +151,21 @@ static void make_partitions(Graph &g, uint32_t part_edges, std::vector<uint32_t>
   if (g.n == 0) pv.assign({0, 0});
 }
 
+// Parent hints pay off only where bottom-up BFS levels run over a split
+// store; they take n x 4 bytes of the budget, so the budget must leave at
+// least as much again for partitions.
+bool store_wants_hints(const Graph &g, uint64_t budget) {
+  return g.sym && g.dir_opt && g.parent_hints && g.n > 0 &&
+         budget != UINT64_MAX && budget < g.m * (uint64_t)g.rec && budget >= 2 * 4 * g.n;
+}
+
+// Options that decide the hints changed: rebuild with the same budget.
+void store_refresh(Graph &g) {
+  Store &s = g.store;
+  if (s.configured && store_wants_hints(g, s.budget) != s.hint_reserved)
+    store_configure(g, s.budget, s.part_edges);
+}
+
 void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   if (part_edges == 0) fail(THV_EINVAL, "partition_edges must be > 0");
   if (g.n >= 0xFFFFFFFFull) fail(THV_ENOTSUP, "graphs need fewer than 2^32-1 vertices");
@@ -159,6 +174,9 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   s.configured = false;
   s.part_edges = part_edges;
   s.budget = budget;
+  s.hint_reserved = store_wants_hints(g, budget);
+  s.d_hint.release();
+  if (s.hint_reserved) budget -= 4 * g.n;  // the hints' share of the budget
   make_partitions(g, part_edges, s.pv);
   s.P = (uint32_t)(s.pv.size() - 1);
   const uint64_t rec = (uint64_t)g.rec;
@@ -248,6 +266,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   HIPC(hipMemcpy(s.d_pv.p, s.pv.data(), (s.P + 1) * 4, hipMemcpyHostToDevice));
   s.d_pres.alloc(s.P ? s.P : 1);
   if (s.P) HIPC(hipMemcpy(s.d_pres.p, s.pres.data(), s.P, hipMemcpyHostToDevice));
+  if (s.hint_reserved && s.host_vertices > 0) build_parent_hints(g);
   s.configured = true;
 }
 

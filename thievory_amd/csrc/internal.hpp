@@ -83,7 +83,14 @@ struct Store {
   DevMem d_pv, d_pres, d_eptr;
   DevMem d_rbit;                     // n bits: vertex's edges are in HBM
   DevMem d_vpart;                    // n x u32: partition of each vertex
+  // Parent hints (symmetric graphs, split store): n x u32, for a vertex
+  // whose list is in host memory its neighbour of largest degree, else
+  // NO_HINT.  Their bytes come out of the budget.
+  bool hint_reserved = false;
+  DevMem d_hint;
 };
+
+constexpr uint32_t NO_HINT = 0xFFFFFFFFu;
 
 // ---- workspace (per graph, allocated on first use) --------------------------
 
@@ -98,7 +105,9 @@ struct LevelCtr {
   unsigned long long rd[2];   // bottom-up BFS: edge records read (host, HBM)
 };
 
-enum { CTR_SLOTS = 3 };       // [0], [1]: alternating levels; [2]: bottom-up queue
+// [0], [1]: alternating levels; [2]: bottom-up early-exit queue; [3]:
+// bottom-up full-scan queue
+enum { CTR_SLOTS = 4 };
 
 struct Graph;
 
@@ -165,8 +174,12 @@ struct Graph {
   double do_alpha = 14.0, do_beta = 24.0;
   // link cost model of a bottom-up level over a split store: each unvisited
   // vertex with host-resident edges costs one small zero-copy read
-  double do_vertex_ns = 1.5;  // per unvisited host vertex (measured on MI355X)
+  double do_vertex_ns = 3.0;  // per long unvisited host list (measured on MI355X)
   double do_link_gbps = 55.0; // sustained zero-copy read rate
+  // bottom-up levels: unvisited vertices of at most this degree are scanned
+  // in full by the edge-balanced kernel, longer lists by the early-exit one
+  uint32_t do_pull_degree = 16;
+  bool parent_hints = true;
   ~Graph();
 };
 
@@ -175,11 +188,16 @@ void graph_init_device(Graph &g);
 void graph_build_tables(Graph &g);
 void store_configure(Graph &g, uint64_t budget, uint32_t part_edges);
 void store_ensure(Graph &g);
+bool store_wants_hints(const Graph &g, uint64_t budget);
+void store_refresh(Graph &g);
 void workspace_ensure(Graph &g);
 uint64_t now_ns();
 
 // kernels (frontier.hip / expand.hip / algos.hip)
-enum Algo { A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4 };
+// A_BFS_PULL: bottom-up BFS as an edge-balanced scan -- the queue holds
+// unvisited vertices; one of them joins the next frontier when any of its
+// edges leads into the current frontier.
+enum Algo { A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_BFS_PULL = 5 };
 
 struct ExpandArgs {
   const uint32_t *qv[2];
@@ -190,6 +208,7 @@ struct ExpandArgs {
   uint32_t *val;            // BFS/CC/SSSP values
   uint32_t *vis;            // BFS visited bitmap
   uint32_t *next;           // next frontier bitmap
+  const uint32_t *front;    // BFS pull: current frontier bitmap
   float *pr_src;            // PR push: delta; PR pull: contribution
   float *pr_dst;            // PR push: residual; PR pull: sum
   uint32_t level_value;     // BFS: value given to discovered vertices
@@ -201,8 +220,12 @@ struct ExpandArgs {
 // unvisited set of a bottom-up BFS level), NO_CHUNKS skips the chunk-start
 // table, COUNT_ONLY stops after the totals.
 enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8 };
+// val (BFS only): every vertex with its bit set gets val_value, written in
+// the counting pass.
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
-                    hipStream_t s, int mode = 0);
+                    hipStream_t s, int mode = 0, uint32_t *val = nullptr,
+                    uint32_t val_value = 0, uint32_t dmin = 1,
+                    uint32_t dmax = 0xFFFFFFFFu);
 
 struct BottomUpArgs {
   const uint32_t *qv[2];
@@ -215,6 +238,11 @@ struct BottomUpArgs {
   uint32_t level_value;
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
+void build_parent_hints(Graph &g);
+// every unvisited vertex whose hint is in front joins vis and next (plain
+// word writes: nothing else may write vis / next meanwhile); hits summed
+void launch_hint_pass(Graph &g, const uint32_t *front, uint32_t *vis, uint32_t *next,
+                      unsigned long long *hits, hipStream_t s);
 ExpandArgs expand_args(const Graph &g, const QueueSet &qs);
 inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);
