@@ -141,9 +141,8 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       lists x DO_VERTEX_NS).
  *   THV_OPT_PARENT_HINTS  1 (default): on a split store of a symmetric graph
  *                       with direction-optimising BFS, n x 4 bytes of the
- *                       HBM budget hold one parent hint per vertex -- the
- *                       neighbour of largest degree of every vertex whose
- *                       list is in host memory.  A bottom-up level first
+ *                       HBM budget hold one parent hint per vertex -- its
+ *                       neighbour of largest degree.  A bottom-up level first
  *                       tests the hints; a vertex whose hint is in the
  *                       frontier joins the next level without its list
  *                       being read.  0: no hints, the whole budget holds

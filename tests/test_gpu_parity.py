@@ -141,13 +141,21 @@ def test_bfs_direction_optimizing(rmat16, frac):
     g.set_option("do_alpha", 14.0)
 
 
+@pytest.fixture(scope="module")
+def rmat16u():
+    """Symmetric R-MAT 16 with 4-byte records: the bottom-up layout (parent
+    hints, sparsest partitions first) applies to it."""
+    return T.Graph.rmat(16, 16, seed=1), O.rmat(16, 16, seed=1)
+
+
 @pytest.mark.parametrize("frac", [1.0, 0.25, 0.0])
 @pytest.mark.parametrize("pull_degree", [0, 1, 16, 1024])
-def test_bfs_bottom_up_full_scan_split(rmat16, frac, pull_degree):
+@pytest.mark.parametrize("records", [4, 8])
+def test_bfs_bottom_up_full_scan_split(rmat16, rmat16u, frac, pull_degree, records):
     """Bottom-up levels split between the full-list scan (degree <= D) and
     the early-exit scan give the same levels for every split point; a slow
     link rate makes the split-store rule choose bottom-up at this size."""
-    g, og = rmat16
+    g, og = rmat16u if records == 4 else rmat16
     _say("case: hybrid bottom-up", frac, pull_degree)
     g.store_config(int(frac * g.m * g.edge_bytes), 4096)
     g.set_option("do_pull_degree", pull_degree)
@@ -169,12 +177,15 @@ def test_bfs_bottom_up_full_scan_split(rmat16, frac, pull_degree):
     assert g.get_option("do_pull_degree") == 16.0
 
 
-def test_parent_hints(rmat16):
+def test_parent_hints(rmat16, rmat16u):
     """Parent hints take n x 4 bytes of the budget on a split store of a
-    symmetric graph, place vertices without reading their lists, and leave
-    the levels unchanged; switching them off gives the bytes back to the
-    partitions."""
-    g, og = rmat16
+    symmetric graph with 4-byte records, place vertices without reading
+    their lists, and leave the levels unchanged; switching them off gives
+    the bytes back to the partitions.  8-byte records get no hints."""
+    gw, _ = rmat16
+    gw.store_config(int(0.25 * gw.m * gw.edge_bytes), 4096)
+    assert gw.get_option("hint_bytes") == 0
+    g, og = rmat16u
     _say("case: parent hints")
     budget = int(0.25 * g.m * g.edge_bytes)
     g.store_config(budget, 4096)

@@ -287,40 +287,35 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     seen += nf;
     visited_edges += mf;
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
-    // probed: the unvisited set was counted, so the tile tables now hold
-    // the short-list count (long first, short last) and the frontier's
-    // write pass needs a fresh count
+    // probed: the bottom-up probe ran (tile set 0 now holds the short
+    // unvisited lists, so a top-down level needs a fresh frontier count)
     bool probed = false, hinted = false;
     double td_cost = 0, bu_cost = 0;
     HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
+    const uint32_t *hint = hints ? g.store.d_hint.as<uint32_t>() : nullptr;
+    uint32_t *next = w.front[cur ^ 1].as<uint32_t>();
     if (host_share) {
       // top-down streams the frontier's host records over the link; the
       // hybrid bottom-up level streams the short host lists in full and
       // pays one small zero-copy read per long unvisited host list.  The
-      // two count passes (about 6 HBM bytes per vertex and a launch +
-      // readback each) run only when the frontier's host records cost more.
+      // probe (hints + counts of the unvisited set, about 6 HBM bytes per
+      // vertex and two readbacks) runs only when the frontier's host
+      // records cost more.
       td_cost = (double)h.qw[0] * rec / link;
       const double probe_cost = 2.0 * ((double)g.n * 6.0 / 5e12 + 15e-6);
       bu = false;
       if (td_cost > probe_cost) {
-        if (hints) {
-          launch_hint_pass(g, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(),
-                           w.front[cur ^ 1].as<uint32_t>(), d_hits, s);
-          hinted = true;
-        }
-        launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_COUNT_ONLY, nullptr, 0,
-                       D + 1, 0xFFFFFFFFu);
-        if (D > 0)
-          launch_compact(g, w.qs, vis, c_short, s, CMP_INVERT | CMP_COUNT_ONLY, nullptr, 0, 1, D);
+        launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint,
+                        D, c_short, c_long, d_hits, s);
         HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
         if (trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_hits, 8, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
-        const uint64_t short_host = D > 0 ? w.h_ctr[3].qw[0] : 0;
-        const uint64_t short_host_n = D > 0 ? w.h_ctr[3].qn[0] : 0;
-        bu_cost = (double)short_host * rec / link + (double)short_host_n * SHORT_NS * 1e-9 +
-                  (double)w.h_ctr[2].qn[0] * g.do_vertex_ns * 1e-9;
+        const LevelCtr &pl = w.h_ctr[2], &ps = w.h_ctr[3];
+        bu_cost = (double)ps.qw[0] * rec / link + (double)ps.qn[0] * SHORT_NS * 1e-9 +
+                  (double)pl.qn[0] * g.do_vertex_ns * 1e-9;
         bu = bu_cost < td_cost;
         probed = true;
+        hinted = hints;
       }
     } else {
       // everything in HBM: the edge-ratio rule of Beamer et al.
@@ -328,9 +323,10 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       else if (bu && (double)nf < (double)g.n / g.do_beta && nf < prev_nf) bu = false;
     }
     prev_nf = nf;
-    a.next = w.front[cur ^ 1].as<uint32_t>();
+    a.next = next;
     a.front = w.front[cur].as<uint32_t>();
     if (!bu) {
+      // the probe replaced the tile counts: a full compaction of the frontier
       launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, probed ? 0 : CMP_WRITE_ONLY);
       a.ctr = c;
       a.level_value = L + 2;
@@ -338,18 +334,20 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       launch_expand(g, A_BFS, a, s);
       HIPC(hipEventRecord(w.ev[1], s));
     } else {
+      if (!probed)
+        launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint,
+                        D, c_short, c_long, d_hits, s);
       HIPC(hipEventRecord(w.ev[0], s));
       if (D > 0) {
-        launch_compact(g, w.qs, vis, c_short, s, CMP_INVERT | (probed ? CMP_WRITE_ONLY : 0),
-                       nullptr, 0, 1, D);
+        launch_compact(g, w.qs, vis, c_short, s, CMP_INVERT | CMP_WRITE_ONLY, nullptr, 0, 1, D, 0);
         ExpandArgs p = a;
         p.ctr = c_short;
         launch_expand(g, A_BFS_PULL, p, s);
       }
       // vertices found by the full scan are visited now but are short, so
-      // the long set is the same as when it was counted
-      launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_NO_CHUNKS, nullptr, 0, D + 1,
-                     0xFFFFFFFFu);
+      // the long set is the one the probe counted
+      launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_NO_CHUNKS | CMP_WRITE_ONLY, nullptr,
+                     0, D + 1, 0xFFFFFFFFu, 1);
       BottomUpArgs b;
       b.qv[0] = w.qs.qv[0].as<uint32_t>();
       b.qv[1] = w.qs.qv[1].as<uint32_t>();
@@ -358,7 +356,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       b.deg = g.d_deg.as<uint32_t>();
       b.front = w.front[cur].as<uint32_t>();
       b.vis = w.vis.as<uint32_t>();
-      b.next = w.front[cur ^ 1].as<uint32_t>();
+      b.next = next;
       b.val = w.val.as<uint32_t>();
       b.rd_ctr = c_long;
       b.level_value = L + 2;

@@ -70,38 +70,14 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
   }
 }
 
-// Parent hints: a group of 16 lanes per vertex whose list is in host
-// memory scans the list (zero-copy) for the neighbour of largest degree,
-// ties to the smallest id.  BFS reaches high-degree vertices early, so in
-// a bottom-up level this one HBM word usually names a frontier neighbour.
-// Built once per store configuration, outside any traversal.
-template <int REC>
-__global__ void __launch_bounds__(BBS) k_hint_build(const uint64_t *eptr, const uint32_t *deg,
-                                                   const uint32_t *rbit, uint64_t n,
-                                                   uint32_t *hint) {
-  const uint32_t gl = threadIdx.x & (G - 1);
-  const unsigned long long ngroups = (unsigned long long)gridDim.x * BBS / G;
-  for (unsigned long long v = (blockIdx.x * (unsigned long long)BBS + threadIdx.x) / G; v < n;
-       v += ngroups) {
-    // v, d and host are the same on the 16 lanes of a group, so the
-    // group's lanes reach the shuffles together
-    const uint32_t d = deg[v];
-    const bool host = d != 0 && !((rbit[v >> 5] >> (v & 31)) & 1u);
-    unsigned long long best = 0;  // (degree << 32) | (~id): max = largest degree, smallest id
-    if (host) {
-      const unsigned long long base = eptr[v];
-      for (uint32_t k = gl; k < d; k += G) {
-        const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (unsigned long long)k * REC);
-        const unsigned long long key = ((unsigned long long)deg[w] << 32) | (0xFFFFFFFFu - w);
-        best = best > key ? best : key;
-      }
-#pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) {
-        const unsigned long long other = __shfl_xor(best, o, 64);
-        best = best > other ? best : other;
-      }
-    }
-    if (gl == 0) hint[v] = host ? 0xFFFFFFFFu - (uint32_t)best : NO_HINT;
+// Parent hints from the keys of the hint-build expansion: the low word of
+// a key is ~(neighbour of largest degree, smallest id among ties).
+__global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *key, uint64_t n,
+                                                    uint32_t *hint) {
+  for (uint64_t v = blockIdx.x * (uint64_t)BBS + threadIdx.x; v < n;
+       v += (uint64_t)gridDim.x * BBS) {
+    const unsigned long long k = key[v];
+    hint[v] = k ? 0xFFFFFFFFu - (uint32_t)k : NO_HINT;
   }
 }
 
@@ -160,21 +136,32 @@ void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
   HIPC(hipGetLastError());
 }
 
+// Parent hints: for every vertex with edges, its neighbour of largest
+// degree (ties: smallest id).  BFS reaches high-degree vertices early, so in
+// a bottom-up level this one HBM word usually names a frontier neighbour.
+// One edge-balanced expansion over every list (HBM and zero-copy host
+// records alike), once per store configuration, outside any traversal.
 void build_parent_hints(Graph &g) {
-  Store &st = g.store;
-  st.d_hint.alloc(g.n * 4);
-  const int grid = 8 * g.cus;
-  const uint64_t *eptr = st.d_eptr.as<uint64_t>();
-  const uint32_t *deg = g.d_deg.as<uint32_t>(), *rbit = st.d_rbit.as<uint32_t>();
-  uint32_t *hint = st.d_hint.as<uint32_t>();
-  if (g.rec == 8)
-    hipLaunchKernelGGL(k_hint_build<8>, dim3(grid), dim3(BBS), 0, g.stream, eptr, deg, rbit, g.n,
-                       hint);
-  else
-    hipLaunchKernelGGL(k_hint_build<4>, dim3(grid), dim3(BBS), 0, g.stream, eptr, deg, rbit, g.n,
-                       hint);
+  Workspace &w = g.ws;
+  workspace_ensure(g);
+  hipStream_t s = g.stream;
+  DevMem key;
+  key.alloc(g.n * 8);
+  HIPC(hipMemsetAsync(key.p, 0, g.n * 8, s));
+  HIPC(hipMemsetAsync(w.front[0].p, 0, bitmap_words(g.n) * 4, s));
+  LevelCtr *c = w.qs.ctr.as<LevelCtr>();
+  launch_compact(g, w.qs, w.front[0].as<uint32_t>(), c, s, CMP_INVERT);  // every vertex
+  ExpandArgs a = expand_args(g, w.qs);
+  a.ctr = c;
+  a.deg = g.d_deg.as<uint32_t>();
+  a.key64 = key.as<unsigned long long>();
+  launch_expand(g, A_HINT, a, s);
+  g.store.d_hint.alloc(g.n * 4);
+  const int grid = (int)std::min<uint64_t>(div_up(g.n, BBS), 16ull * g.cus);
+  hipLaunchKernelGGL(k_hint_finish, dim3(grid ? grid : 1), dim3(BBS), 0, s,
+                     key.as<unsigned long long>(), g.n, g.store.d_hint.as<uint32_t>());
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(g.stream));
+  HIPC(hipStreamSynchronize(s));
 }
 
 void launch_hint_pass(Graph &g, const uint32_t *front, uint32_t *vis, uint32_t *next,

@@ -32,6 +32,8 @@
 //   BFS pull (bottom-up level on a symmetric graph, queue = unvisited
 //          vertices): an entry whose list holds a frontier vertex joins the
 //          next frontier; same levels as the top-down rule
+//   HINT  (store set-up, not a traversal): key64[v] = max over the edges
+//          of (deg[w] << 32 | ~w), reduced per entry in LDS
 #include "chunk.hpp"
 
 namespace thv {
@@ -75,10 +77,8 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
           s_u[t] = v;
           s_f[t] = 0.0f;
         }
-        if (ALGO == A_BFS_PULL) {
-          s_u[t] = v;
-          s_hit[t] = 0;
-        }
+        if (ALGO == A_BFS_PULL || ALGO == A_HINT) s_u[t] = v;
+        if (ALGO == A_BFS_PULL) s_hit[t] = 0;
       }
     }
     __syncthreads();
@@ -106,6 +106,14 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       }
     }
 
+    if (ALGO == A_HINT) {
+      // every thread's records are in registers: s_base becomes the
+      // entries' running maxima
+      __syncthreads();
+      for (uint32_t t = tid; t < ci.nent; t += EBS) s_base[t] = 0;
+      __syncthreads();
+    }
+
 #pragma unroll
     for (int j = 0; j < EPT; j++) {
       const uint32_t w = dst[j];
@@ -128,6 +136,8 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
         }
       } else if (ALGO == A_BFS_PULL) {
         if (a.front[wd] & bit) s_hit[ent[j]] = 1;
+      } else if (ALGO == A_HINT) {
+        atomicMax(&s_base[ent[j]], ((unsigned long long)a.deg[w] << 32) | (0xFFFFFFFFu - w));
       } else if (ALGO == A_PR_PUSH) {
         unsafeAtomicAdd(&a.pr_dst[w], s_f[ent[j]]);
       } else {  // A_PR_PULL: w is an in-neighbour of the entry's vertex
@@ -138,6 +148,13 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       __syncthreads();
       for (uint32_t t = tid; t < ci.nent; t += EBS) {
         if (chunk_rel(qe[ci.i0 + t], ci.e0) < C) unsafeAtomicAdd(&a.pr_dst[s_u[t]], s_f[t]);
+      }
+    }
+    if (ALGO == A_HINT) {
+      __syncthreads();
+      for (uint32_t t = tid; t < ci.nent; t += EBS) {
+        if (chunk_rel(qe[ci.i0 + t], ci.e0) < C && s_base[t])
+          atomicMax(&a.key64[s_u[t]], s_base[t]);
       }
     }
     if (ALGO == A_BFS_PULL) {
@@ -190,6 +207,9 @@ void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
       break;
     case A_BFS_PULL:
       r8 ? launch_t<A_BFS_PULL, 8>(g, args, s) : launch_t<A_BFS_PULL, 4>(g, args, s);
+      break;
+    case A_HINT:
+      r8 ? launch_t<A_HINT, 8>(g, args, s) : launch_t<A_HINT, 4>(g, args, s);
       break;
   }
   HIPC(hipGetLastError());

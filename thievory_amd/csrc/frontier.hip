@@ -11,14 +11,17 @@
 //   k_tscan   one block: exclusive tile bases per queue, totals
 //   k_twrite  one block per tile with work: ordered queue entries (vertex
 //             id, first edge in the level's work space), the chunk-start
-//             table the expansion kernel indexes by chunk, and a count of
-//             the partitions touched (stamped, so each counts once)
+//             table the expansion kernel indexes by chunk, and a stamp on
+//             every partition touched
+//   k_touched the stamped partitions, counted per residency
 // A host partition with no active vertex produces no entry, so its edges are
 // never read (the per-partition activity filter).  Queues come out sorted by
 // vertex id, which makes the zero-copy reads of a dense host frontier
 // sequential.  A wave reads its sub-tile's bitmap words with one load and
 // the degrees of all its selected vertices with one batch of loads, so an
 // empty sub-tile costs one load and a few lane shuffles.
+#include <algorithm>
+
 #include "internal.hpp"
 #include "scan.hpp"
 
@@ -32,8 +35,10 @@ constexpr uint32_t ROUNDS = SUB / 64;
 
 struct CompactArgs {
   const uint32_t *front, *rbit, *deg, *vpart;
+  const uint8_t *pres;
   uint64_t n;
   uint32_t ntiles;
+  uint32_t P;
   uint32_t *tcnt, *tbc;
   unsigned long long *tdeg, *tbe;
   uint32_t *qv[2];
@@ -211,83 +216,199 @@ __global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
   }
 }
 
+// The write pass gives each lane a run of VPL consecutive vertices (the
+// count pass gives it one vertex per round): the entries a lane writes are
+// consecutive, so one wave scan of the lanes' totals places them all, in
+// place of a scan per 64 vertices.  It selects exactly the vertices the
+// count pass counted.
+constexpr uint32_t VPL = SUB / 64;
+static_assert(VPL == 16, "a lane's run is half a bitmap word");
+
 __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
   __shared__ unsigned long long s[CBS / 64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const unsigned long long lt = lanemask_lt();
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     if (a.tcnt[2 * tile] == 0 && a.tcnt[2 * tile + 1] == 0) continue;
-    const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB;
-    SubTile t;
-    load_subtile(a, base, lane, t);
-    // partition ids of the selected vertices, loaded together
-    uint32_t part[ROUNDS];
+    const uint64_t v0 = (uint64_t)tile * TILE + (uint64_t)wv * SUB + (uint64_t)lane * VPL;
+    const uint32_t sh = (uint32_t)(v0 & 31);
+    uint32_t sel = 0, dev = 0;
+    if (v0 < a.n) {
+      const uint32_t fw = (a.front[v0 >> 5] >> sh) & 0xFFFFu;
+      dev = (a.rbit[v0 >> 5] >> sh) & 0xFFFFu;
+      sel = (a.invert ? ~fw : fw) & 0xFFFFu;
+      if (a.n - v0 < VPL) sel &= (1u << (a.n - v0)) - 1u;
+    }
+    uint32_t d[VPL], part[VPL];
 #pragma unroll
-    for (uint32_t r = 0; r < ROUNDS; r++)
-      part[r] = ((t.act >> r) & 1u) ? a.vpart[base + r * 64 + lane] : 0xFFFFFFFFu;
-    // this wave's share, to place it after the waves before it
-    uint32_t c[2];
-    unsigned long long e[2];
-    subtile_sums(t, c, e);
-    const unsigned long long w0 = wave_sum((unsigned long long)c[0]);
-    const unsigned long long w1 = wave_sum((unsigned long long)c[1]);
-    const unsigned long long w2 = wave_sum(e[0]);
-    const unsigned long long w3 = wave_sum(e[1]);
-    if (lane == 0) {
-      s[wv][0] = w0;
-      s[wv][1] = w1;
-      s[wv][2] = w2;
-      s[wv][3] = w3;
+    for (uint32_t r = 0; r < VPL; r++) d[r] = ((sel >> r) & 1u) ? a.deg[v0 + r] : 0u;
+#pragma unroll
+    for (uint32_t r = 0; r < VPL; r++) {
+      if (d[r] < a.dmin || d[r] > a.dmax) {
+        sel &= ~(1u << r);
+        d[r] = 0;
+      }
+    }
+    dev &= sel;
+#pragma unroll
+    for (uint32_t r = 0; r < VPL; r++)
+      part[r] = ((sel >> r) & 1u) ? a.vpart[v0 + r] : 0xFFFFFFFFu;
+    // lane totals -> exclusive offsets within the wave, then the block
+    const unsigned long long cd = __popc(dev), ch = __popc(sel) - cd;
+    unsigned long long ed = 0, eh = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < VPL; r++) {
+      if ((dev >> r) & 1u) ed += d[r];
+      else eh += d[r];
+    }
+    const unsigned long long ich = wave_incl_scan(ch), icd = wave_incl_scan(cd);
+    const unsigned long long ieh = wave_incl_scan(eh), ied = wave_incl_scan(ed);
+    if (lane == 63) {
+      s[wv][0] = ich;
+      s[wv][1] = icd;
+      s[wv][2] = ieh;
+      s[wv][3] = ied;
     }
     __syncthreads();
-    unsigned long long cb[2] = {a.tbc[2 * tile], a.tbc[2 * tile + 1]};
-    unsigned long long eb[2] = {a.tbe[2 * tile], a.tbe[2 * tile + 1]};
+    unsigned long long idx[2] = {a.tbc[2 * tile] + ich - ch, a.tbc[2 * tile + 1] + icd - cd};
+    unsigned long long eb[2] = {a.tbe[2 * tile] + ieh - eh, a.tbe[2 * tile + 1] + ied - ed};
     for (uint32_t w = 0; w < wv; w++) {
-      cb[0] += s[w][0];
-      cb[1] += s[w][1];
+      idx[0] += s[w][0];
+      idx[1] += s[w][1];
       eb[0] += s[w][2];
       eb[1] += s[w][3];
     }
-    // ordered entries
+    uint32_t prevp = 0xFFFFFFFFu;
 #pragma unroll
-    for (uint32_t r = 0; r < ROUNDS; r++) {
-      const uint64_t v = base + r * 64 + lane;
-      const bool act = (t.act >> r) & 1u;
-      const int q = ((t.dev >> r) & 1u) ? Q_DEV : Q_HOST;
-      const uint32_t d = t.d[r];
-      const unsigned long long bh = __ballot(act && q == Q_HOST);
-      const unsigned long long bd = __ballot(act && q == Q_DEV);
-      if (!(bh | bd)) continue;
-      const unsigned long long xh = (act && q == Q_HOST) ? d : 0ull;
-      const unsigned long long xd = (act && q == Q_DEV) ? d : 0ull;
-      const unsigned long long ih = wave_incl_scan(xh), id = wave_incl_scan(xd);
-      if (act) {
-        const unsigned long long idx = cb[q] + __popcll((q == Q_HOST ? bh : bd) & lt);
-        const unsigned long long e0 = eb[q] + (q == Q_HOST ? ih - xh : id - xd);
-        a.qv[q][idx] = (uint32_t)v;
-        a.qe[q][idx] = e0;
-        if (!a.no_chunks) {
-          uint32_t *qc = a.qc[q];
-          for (unsigned long long ch = (e0 + a.C - 1) / a.C; ch * a.C < e0 + d; ch++)
-            qc[ch] = (uint32_t)idx;
-        }
+    for (uint32_t r = 0; r < VPL; r++) {
+      if (!((sel >> r) & 1u)) continue;
+      const int q = ((dev >> r) & 1u) ? Q_DEV : Q_HOST;
+      const unsigned long long i = idx[q]++, e0 = eb[q];
+      eb[q] += d[r];
+      a.qv[q][i] = (uint32_t)(v0 + r);
+      a.qe[q][i] = e0;
+      if (!a.no_chunks) {
+        uint32_t *qc = a.qc[q];
+        for (unsigned long long c = (e0 + a.C - 1) / a.C; c * a.C < e0 + d[r]; c++)
+          qc[c] = (uint32_t)i;
       }
-      // partitions touched: among the active lanes, the first of every run
-      // of equal partition ids stamps it; a stamp that changes counts the
-      // partition once
-      const uint32_t p = part[r];
-      const unsigned long long below = (bh | bd) & lt;
-      const int prev = below ? 63 - __clzll((long long)below) : (int)lane;
-      const uint32_t pp = (uint32_t)__shfl((int)p, prev, 64);
-      if (act && (!below || pp != p)) {
-        if (atomicExch(&a.ptouch[p], a.stamp) != a.stamp) atomicAdd(&a.ctr->touched[q], 1u);
+      // partitions touched: a plain store of this call's stamp wherever
+      // the lane's partition changes (every writer writes the same value);
+      // k_touched counts the stamps afterwards
+      if (part[r] != prevp) {
+        a.ptouch[part[r]] = a.stamp;
+        prevp = part[r];
       }
-      cb[0] += __popcll(bh);
-      cb[1] += __popcll(bd);
-      eb[0] += __shfl(ih, 63, 64);
-      eb[1] += __shfl(id, 63, 64);
     }
     __syncthreads();
+  }
+}
+
+struct ProbeArgs {
+  const uint32_t *front, *rbit, *deg, *hint;
+  uint32_t *vis, *next;
+  uint64_t n;
+  uint32_t ntiles, D;
+  uint32_t *tcnt[2];
+  unsigned long long *tdeg[2];
+  unsigned long long *hits;
+};
+
+// Bottom-up probe: one wave per 1024-vertex sub-tile, three batches of
+// loads (visited and residency words; hints and their frontier bits;
+// degrees of the vertices the hints did not place).  Hint hits are written
+// into the wave's own visited / next words by the lanes that loaded them.
+__global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
+  __shared__ unsigned long long s[CBS / 64][8];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  unsigned long long hits = 0;
+  for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB, w0 = base >> 5;
+    const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
+    const uint32_t vw = owner ? a.vis[w0 + lane] : 0xFFFFFFFFu;
+    const uint32_t rw = owner ? a.rbit[w0 + lane] : 0u;
+    uint32_t un = 0, dev = 0;  // bit r: round r's vertex is unvisited / HBM-resident
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+      const int src = (int)(2 * r + (lane >> 5));
+      const uint32_t bit = 1u << (lane & 31);
+      const bool in = base + r * 64 + lane < a.n;
+      if (in && !((uint32_t)__shfl((int)vw, src, 64) & bit)) un |= 1u << r;
+      if ((uint32_t)__shfl((int)rw, src, 64) & bit) dev |= 1u << r;
+    }
+    if (a.hint) {
+      uint32_t h[ROUNDS];
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++)
+        h[r] = ((un >> r) & 1u) ? a.hint[base + r * 64 + lane] : NO_HINT;
+      uint32_t f = 0;
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++)
+        if (h[r] != NO_HINT && ((a.front[h[r] >> 5] >> (h[r] & 31)) & 1u)) f |= 1u << r;
+      uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++) {
+        const unsigned long long b = __ballot((f >> r) & 1u);
+        if (lane == 2 * r) m = (uint32_t)b;
+        if (lane == 2 * r + 1) m = (uint32_t)(b >> 32);
+      }
+      if (owner && m) {
+        a.vis[w0 + lane] = vw | m;
+        a.next[w0 + lane] |= m;
+        hits += __popc(m);
+      }
+      un &= ~f;
+    }
+    uint32_t d[ROUNDS];
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
+    // sums: [0..3] short set (host n, dev n, host e, dev e), [4..7] long set
+    unsigned long long x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+      const uint32_t q = (dev >> r) & 1u;
+      if (d[r] == 0) continue;
+      const uint32_t k = d[r] <= a.D ? 0 : 4;
+      x[k + q] += 1;
+      x[k + 2 + q] += d[r];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = wave_sum(x[i]);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) s[wv][i] = x[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+      unsigned long long y = 0;
+      for (int w = 0; w < CBS / 64; w++) y += s[w][threadIdx.x];
+      const uint32_t set = threadIdx.x >> 2, i = threadIdx.x & 3;
+      if (i < 2) a.tcnt[set][2 * tile + i] = (uint32_t)y;
+      else a.tdeg[set][2 * tile + i - 2] = y;
+    }
+    __syncthreads();
+  }
+  // one atomic per block (the grid is a few blocks per CU)
+  hits = wave_sum(hits);
+  if (lane == 0) s[wv][0] = hits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < CBS / 64; w++) t += s[w][0];
+    if (t) atomicAdd(a.hits, t);
+  }
+}
+
+// Partitions stamped by this call, per residency.
+__global__ void __launch_bounds__(CBS) k_touched(CompactArgs a) {
+  __shared__ unsigned long long sh[CBS / 64];
+  uint32_t c[2] = {0, 0};
+  for (uint32_t p = blockIdx.x * CBS + threadIdx.x; p < a.P; p += gridDim.x * CBS)
+    if (a.ptouch[p] == a.stamp) c[a.pres[p] ? Q_DEV : Q_HOST]++;
+  const unsigned long long h = block_sum<CBS>((unsigned long long)c[0], sh);
+  const unsigned long long d = block_sum<CBS>((unsigned long long)c[1], sh);
+  if (threadIdx.x == 0) {
+    if (h) atomicAdd(&a.ctr->touched[Q_HOST], (unsigned)h);
+    if (d) atomicAdd(&a.ctr->touched[Q_DEV], (unsigned)d);
   }
 }
 
@@ -295,7 +416,7 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
 
 void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode, uint32_t *val, uint32_t val_value, uint32_t dmin,
-                    uint32_t dmax) {
+                    uint32_t dmax, int tset) {
   if (dmin == 0) dmin = 1;  // degree-0 vertices never have work
   Store &st = g.store;
   CompactArgs a;
@@ -304,12 +425,14 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   a.rbit = st.d_rbit.as<uint32_t>();
   a.deg = g.d_deg.as<uint32_t>();
   a.vpart = st.d_vpart.as<uint32_t>();
+  a.pres = st.d_pres.as<uint8_t>();
+  a.P = st.P;
   a.n = g.n;
   a.ntiles = (uint32_t)(g.n / TILE + 1);
-  a.tcnt = w.tcnt.as<uint32_t>();
-  a.tdeg = w.tdeg.as<unsigned long long>();
-  a.tbc = w.tbc.as<uint32_t>();
-  a.tbe = w.tbe.as<unsigned long long>();
+  a.tcnt = w.tcnt[tset].as<uint32_t>();
+  a.tdeg = w.tdeg[tset].as<unsigned long long>();
+  a.tbc = w.tbc[tset].as<uint32_t>();
+  a.tbe = w.tbe[tset].as<unsigned long long>();
   for (int q = 0; q < 2; q++) {
     a.qv[q] = w.qv[q].as<uint32_t>();
     a.qe[q] = w.qe[q].as<unsigned long long>();
@@ -336,8 +459,47 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
     hipLaunchKernelGGL(k_tcount, dim3(grid), dim3(CBS), 0, s, a);
     hipLaunchKernelGGL(k_tscan, dim3(1), dim3(SBS), 0, s, a);
   }
-  if (!(mode & CMP_COUNT_ONLY))
+  if (!(mode & CMP_COUNT_ONLY)) {
     hipLaunchKernelGGL(k_twrite, dim3(grid), dim3(CBS), 0, s, a);
+    const uint32_t tg = (uint32_t)std::min<uint64_t>(div_up(st.P, CBS), 64);
+    hipLaunchKernelGGL(k_touched, dim3(tg ? tg : 1), dim3(CBS), 0, s, a);
+  }
+  HIPC(hipGetLastError());
+}
+
+void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis,
+                     uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
+                     LevelCtr *cl, unsigned long long *hits, hipStream_t s) {
+  ProbeArgs p;
+  memset(&p, 0, sizeof p);
+  p.front = front;
+  p.rbit = g.store.d_rbit.as<uint32_t>();
+  p.deg = g.d_deg.as<uint32_t>();
+  p.hint = hint;
+  p.vis = vis;
+  p.next = next;
+  p.n = g.n;
+  p.ntiles = (uint32_t)(g.n / TILE + 1);
+  p.D = D;
+  for (int k = 0; k < 2; k++) {
+    p.tcnt[k] = w.tcnt[k].as<uint32_t>();
+    p.tdeg[k] = w.tdeg[k].as<unsigned long long>();
+  }
+  p.hits = hits;
+  const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
+  hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);
+  // tile bases and totals of both sets
+  for (int k = 0; k < 2; k++) {
+    CompactArgs a;
+    memset(&a, 0, sizeof a);
+    a.ntiles = p.ntiles;
+    a.tcnt = w.tcnt[k].as<uint32_t>();
+    a.tdeg = w.tdeg[k].as<unsigned long long>();
+    a.tbc = w.tbc[k].as<uint32_t>();
+    a.tbe = w.tbe[k].as<unsigned long long>();
+    a.ctr = k == 0 ? cs : cl;
+    hipLaunchKernelGGL(k_tscan, dim3(1), dim3(SBS), 0, s, a);
+  }
   HIPC(hipGetLastError());
 }
 

@@ -151,11 +151,14 @@ static void make_partitions(Graph &g, uint32_t part_edges, std::vector<uint32_t>
   if (g.n == 0) pv.assign({0, 0});
 }
 
-// Parent hints pay off only where bottom-up BFS levels run over a split
-// store; they take n x 4 bytes of the budget, so the budget must leave at
-// least as much again for partitions.
+// The bottom-up layout (parent hints + sparsest partitions first) pays off
+// where bottom-up BFS levels run over a split store: a symmetric graph with
+// direction-optimising BFS and 4-byte records (8-byte {dst, weight}
+// records are loaded for SSSP, whose top-down levels want the dense
+// partitions in HBM).  The hints take n x 4 bytes of the budget, so the
+// budget must leave at least as much again for partitions.
 bool store_wants_hints(const Graph &g, uint64_t budget) {
-  return g.sym && g.dir_opt && g.parent_hints && g.n > 0 &&
+  return g.sym && g.dir_opt && g.parent_hints && g.rec == 4 && g.n > 0 &&
          budget != UINT64_MAX && budget < g.m * (uint64_t)g.rec && budget >= 2 * 4 * g.n;
 }
 
@@ -181,17 +184,34 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   s.P = (uint32_t)(s.pv.size() - 1);
   const uint64_t rec = (uint64_t)g.rec;
 
-  // Residency: partitions ranked by average degree (edges per vertex),
-  // densest first, taken whole while the byte budget allows.
-  std::vector<uint32_t> order(s.P);
+  // Residency: partitions ranked by average degree (edges per vertex), taken
+  // whole while the byte budget allows.  Densest first: the hubs, which
+  // top-down levels and repeated relaxations read most, stay in HBM.
+  // Sparsest first under the bottom-up layout: a bottom-up level pays one
+  // host request per unvisited vertex with a host list, so HBM holds the
+  // most vertices per byte, and the hubs, whose lists are read top-down
+  // only from small frontiers or never (hints), stay on the host.
+  // (Sparsest first counts only vertices with edges: isolated vertices
+  // cost no request.)
+  const bool sparse_first = s.hint_reserved;
+  std::vector<uint32_t> order(s.P), pverts(s.P);
   std::iota(order.begin(), order.end(), 0u);
   auto pedges = [&](uint32_t p) { return g.h_off[s.pv[p + 1]] - g.h_off[s.pv[p]]; };
+  for (uint32_t p = 0; p < s.P; p++) {
+    uint32_t c = 0;
+    if (sparse_first) {
+      for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) c += g.h_off[v + 1] != g.h_off[v];
+    } else {
+      c = s.pv[p + 1] - s.pv[p];
+    }
+    pverts[p] = c ? c : 1;
+  }
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
     uint64_t ea = pedges(a), eb = pedges(b);
-    uint64_t va = s.pv[a + 1] - s.pv[a], vb = s.pv[b + 1] - s.pv[b];
+    uint64_t va = pverts[a], vb = pverts[b];
     // ea/va > eb/vb  <=>  ea*vb > eb*va  (128-bit products)
     unsigned __int128 l = (unsigned __int128)ea * vb, r = (unsigned __int128)eb * va;
-    return l > r;
+    return sparse_first ? l < r : l > r;
   });
   s.pres.assign(s.P, 0);
   uint64_t used = 0;
@@ -266,8 +286,8 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   HIPC(hipMemcpy(s.d_pv.p, s.pv.data(), (s.P + 1) * 4, hipMemcpyHostToDevice));
   s.d_pres.alloc(s.P ? s.P : 1);
   if (s.P) HIPC(hipMemcpy(s.d_pres.p, s.pres.data(), s.P, hipMemcpyHostToDevice));
-  if (s.hint_reserved && s.host_vertices > 0) build_parent_hints(g);
   s.configured = true;
+  if (s.hint_reserved && s.host_vertices > 0) build_parent_hints(g);
 }
 
 void QueueSet::ensure(const Graph &g, uint32_t chunk) {
@@ -278,11 +298,13 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
     P = g.store.P;
   }
   const uint64_t tiles = g.n / TILE + 1;
-  if (tcnt.bytes < tiles * 2 * 4) {
-    tcnt.alloc(tiles * 2 * 4);
-    tdeg.alloc(tiles * 2 * 8);
-    tbc.alloc(tiles * 2 * 4);
-    tbe.alloc(tiles * 2 * 8);
+  for (int k = 0; k < 2; k++) {
+    if (tcnt[k].bytes < tiles * 2 * 4) {
+      tcnt[k].alloc(tiles * 2 * 4);
+      tdeg[k].alloc(tiles * 2 * 8);
+      tbc[k].alloc(tiles * 2 * 4);
+      tbe[k].alloc(tiles * 2 * 8);
+    }
   }
   if (n == g.n && m == g.m && C == chunk && ctr.p) return;
   for (int q = 0; q < 2; q++) {

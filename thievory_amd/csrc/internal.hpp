@@ -84,8 +84,8 @@ struct Store {
   DevMem d_rbit;                     // n bits: vertex's edges are in HBM
   DevMem d_vpart;                    // n x u32: partition of each vertex
   // Parent hints (symmetric graphs, split store): n x u32, for a vertex
-  // whose list is in host memory its neighbour of largest degree, else
-  // NO_HINT.  Their bytes come out of the budget.
+  // with edges its neighbour of largest degree, else NO_HINT.  Their bytes
+  // come out of the budget.
   bool hint_reserved = false;
   DevMem d_hint;
 };
@@ -123,8 +123,10 @@ struct QueueSet {
   uint32_t stamp = 0;         // compaction call counter (touch stamps)
   DevMem qv[2], qe[2], qc[2]; // queue vertex ids, edge prefixes, chunk starts
   DevMem ctr;                 // LevelCtr[2]
-  DevMem tcnt, tdeg;          // per tile and queue: active vertices / edges
-  DevMem tbc, tbe;            // per tile and queue: exclusive bases
+  // per tile and queue, in two sets (a bottom-up level counts its short
+  // and long unvisited lists in one sweep):
+  DevMem tcnt[2], tdeg[2];    // active vertices / edges
+  DevMem tbc[2], tbe[2];      // exclusive bases
   DevMem ptouch;              // per partition: stamp of the last touching call
   void ensure(const Graph &g, uint32_t C);
 };
@@ -197,7 +199,11 @@ uint64_t now_ns();
 // A_BFS_PULL: bottom-up BFS as an edge-balanced scan -- the queue holds
 // unvisited vertices; one of them joins the next frontier when any of its
 // edges leads into the current frontier.
-enum Algo { A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_BFS_PULL = 5 };
+// A_HINT: parent-hint build -- key64[v] = max over v's edges (v, w) of
+// (deg[w] << 32 | ~w).
+enum Algo {
+  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_BFS_PULL = 5, A_HINT = 6
+};
 
 struct ExpandArgs {
   const uint32_t *qv[2];
@@ -209,6 +215,8 @@ struct ExpandArgs {
   uint32_t *vis;            // BFS visited bitmap
   uint32_t *next;           // next frontier bitmap
   const uint32_t *front;    // BFS pull: current frontier bitmap
+  const uint32_t *deg;      // hint build: degrees
+  unsigned long long *key64;  // hint build: per-vertex best key
   float *pr_src;            // PR push: delta; PR pull: contribution
   float *pr_dst;            // PR push: residual; PR pull: sum
   uint32_t level_value;     // BFS: value given to discovered vertices
@@ -222,10 +230,20 @@ struct ExpandArgs {
 enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8 };
 // val (BFS only): every vertex with its bit set gets val_value, written in
 // the counting pass.
+// tset: which tile-table set holds the counts (WRITE_ONLY reads it).
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode = 0, uint32_t *val = nullptr,
                     uint32_t val_value = 0, uint32_t dmin = 1,
-                    uint32_t dmax = 0xFFFFFFFFu);
+                    uint32_t dmax = 0xFFFFFFFFu, int tset = 0);
+// Bottom-up probe, one sweep over the vertices: every unvisited vertex
+// whose parent hint (hint may be null) is in front joins vis and next
+// (hits summed into *hits); the other unvisited vertices are counted as
+// two sets for WRITE_ONLY compactions of the inverted vis bitmap: degree
+// 1..D in tile set 0 with totals in *cs, degree > D in tile set 1 with
+// totals in *cl.
+void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vis,
+                     uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
+                     LevelCtr *cl, unsigned long long *hits, hipStream_t s);
 
 struct BottomUpArgs {
   const uint32_t *qv[2];
