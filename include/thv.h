@@ -131,11 +131,10 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       zero-copy read); default 3 (measured on MI355X)
  *   THV_OPT_DO_LINK_GBPS  the link rate of that rule, default 55
  *   THV_OPT_DO_PULL_DEGREE  a bottom-up level scans the whole list of every
- *                       unvisited vertex of at most this degree with the
- *                       edge-balanced kernel (coalesced reads); longer
- *                       lists stop at the first frontier neighbour.
- *                       Default 16; 0 sends every vertex to the early-exit
- *                       scan.  The split-store rule then prices a
+ *                       unvisited vertex of at most this degree (0..16,
+ *                       default 16) inside its probe sweep; longer lists
+ *                       stop at the first frontier neighbour.  0 sends
+ *                       every vertex to the early-exit scan.  The split-store rule then prices a
  *                       bottom-up level as (bytes of those short lists on
  *                       the host / link rate) + (longer unvisited host
  *                       lists x DO_VERTEX_NS).

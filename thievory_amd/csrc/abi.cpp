@@ -121,7 +121,7 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
         g->do_link_gbps = value;
         break;
       case THV_OPT_DO_PULL_DEGREE:
-        NEED(value >= 0.0 && value <= 1024.0);
+        NEED(value >= 0.0 && value <= 16.0);
         g->do_pull_degree = (uint32_t)value;
         break;
       default: fail(THV_EINVAL, "unknown option");

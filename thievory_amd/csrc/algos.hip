@@ -324,7 +324,6 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     }
     prev_nf = nf;
     a.next = next;
-    a.front = w.front[cur].as<uint32_t>();
     if (!bu) {
       // the probe replaced the tile counts: a full compaction of the frontier
       launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, probed ? 0 : CMP_WRITE_ONLY);
@@ -338,14 +337,8 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint,
                         D, c_short, c_long, d_hits, s);
       HIPC(hipEventRecord(w.ev[0], s));
-      if (D > 0) {
-        launch_compact(g, w.qs, vis, c_short, s, CMP_INVERT | CMP_WRITE_ONLY, nullptr, 0, 1, D, 0);
-        ExpandArgs p = a;
-        p.ctr = c_short;
-        launch_expand(g, A_BFS_PULL, p, s);
-      }
-      // vertices found by the full scan are visited now but are short, so
-      // the long set is the one the probe counted
+      // the probe scanned the short lists already; the long set is the one
+      // it counted
       launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_NO_CHUNKS | CMP_WRITE_ONLY, nullptr,
                      0, D + 1, 0xFFFFFFFFu, 1);
       BottomUpArgs b;

@@ -29,9 +29,6 @@
 //   SSSP : atomicMin(val[w], val[v] + weight) (sssp_kernels.cu:124-141)
 //   PR   : push atomicAdd(res[w], delta[v]) (pr_kernels.cu:539-542);
 //          pull sum[v] += contrib[u] over in-edges (pr_kernels.cu:45-51)
-//   BFS pull (bottom-up level on a symmetric graph, queue = unvisited
-//          vertices): an entry whose list holds a frontier vertex joins the
-//          next frontier; same levels as the top-down rule
 //   HINT  (store set-up, not a traversal): key64[v] = max over the edges
 //          of (deg[w] << 32 | ~w), reduced per entry in LDS
 #include "chunk.hpp"
@@ -49,7 +46,6 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   __shared__ unsigned long long s_base[C + 1];
   __shared__ uint32_t s_u[C + 1];  // CC/SSSP: source value; PR pull: vertex id
   __shared__ float s_f[C + 1];     // PR push: delta; PR pull: partial sum
-  uint32_t *s_hit = reinterpret_cast<uint32_t *>(s_f);  // BFS pull: parent found
 
   const ChunkSrc cs = chunk_src(a.ctr, C);
   const uint32_t tid = threadIdx.x;
@@ -77,8 +73,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
           s_u[t] = v;
           s_f[t] = 0.0f;
         }
-        if (ALGO == A_BFS_PULL || ALGO == A_HINT) s_u[t] = v;
-        if (ALGO == A_BFS_PULL) s_hit[t] = 0;
+        if (ALGO == A_HINT) s_u[t] = v;
       }
     }
     __syncthreads();
@@ -134,8 +129,6 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
           const uint32_t old = atomicMin(&a.val[w], nv);
           if (nv < old && !(a.next[wd] & bit)) atomicOr(&a.next[wd], bit);
         }
-      } else if (ALGO == A_BFS_PULL) {
-        if (a.front[wd] & bit) s_hit[ent[j]] = 1;
       } else if (ALGO == A_HINT) {
         atomicMax(&s_base[ent[j]], ((unsigned long long)a.deg[w] << 32) | (0xFFFFFFFFu - w));
       } else if (ALGO == A_PR_PUSH) {
@@ -155,18 +148,6 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       for (uint32_t t = tid; t < ci.nent; t += EBS) {
         if (chunk_rel(qe[ci.i0 + t], ci.e0) < C && s_base[t])
           atomicMax(&a.key64[s_u[t]], s_base[t]);
-      }
-    }
-    if (ALGO == A_BFS_PULL) {
-      // a list split over two chunks may be found twice: both ORs are
-      // idempotent and the next compaction writes the level
-      __syncthreads();
-      for (uint32_t t = tid; t < ci.nent; t += EBS) {
-        if (chunk_rel(qe[ci.i0 + t], ci.e0) < C && s_hit[t]) {
-          const uint32_t v = s_u[t];
-          atomicOr(&a.vis[v >> 5], 1u << (v & 31));
-          atomicOr(&a.next[v >> 5], 1u << (v & 31));
-        }
       }
     }
     __syncthreads();
@@ -204,9 +185,6 @@ void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
       break;
     case A_PR_PULL:
       r8 ? launch_t<A_PR_PULL, 8>(g, args, s) : launch_t<A_PR_PULL, 4>(g, args, s);
-      break;
-    case A_BFS_PULL:
-      r8 ? launch_t<A_BFS_PULL, 8>(g, args, s) : launch_t<A_BFS_PULL, 4>(g, args, s);
       break;
     case A_HINT:
       r8 ? launch_t<A_HINT, 8>(g, args, s) : launch_t<A_HINT, 4>(g, args, s);

@@ -231,6 +231,9 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
     if (a.tcnt[2 * tile] == 0 && a.tcnt[2 * tile + 1] == 0) continue;
     const uint64_t v0 = (uint64_t)tile * TILE + (uint64_t)wv * SUB + (uint64_t)lane * VPL;
     const uint32_t sh = (uint32_t)(v0 & 31);
+    // the tile's bases load together with the words
+    const unsigned long long tb[4] = {a.tbc[2 * tile], a.tbc[2 * tile + 1], a.tbe[2 * tile],
+                                      a.tbe[2 * tile + 1]};
     uint32_t sel = 0, dev = 0;
     if (v0 < a.n) {
       const uint32_t fw = (a.front[v0 >> 5] >> sh) & 0xFFFFu;
@@ -269,8 +272,8 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
       s[wv][3] = ied;
     }
     __syncthreads();
-    unsigned long long idx[2] = {a.tbc[2 * tile] + ich - ch, a.tbc[2 * tile + 1] + icd - cd};
-    unsigned long long eb[2] = {a.tbe[2 * tile] + ieh - eh, a.tbe[2 * tile + 1] + ied - ed};
+    unsigned long long idx[2] = {tb[0] + ich - ch, tb[1] + icd - cd};
+    unsigned long long eb[2] = {tb[2] + ieh - eh, tb[3] + ied - ed};
     for (uint32_t w = 0; w < wv; w++) {
       idx[0] += s[w][0];
       idx[1] += s[w][1];
@@ -305,27 +308,44 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
 
 struct ProbeArgs {
   const uint32_t *front, *rbit, *deg, *hint;
+  const uint64_t *eptr;
   uint32_t *vis, *next;
   uint64_t n;
-  uint32_t ntiles, D;
+  uint32_t ntiles, D, rec;
   uint32_t *tcnt[2];
   unsigned long long *tdeg[2];
   unsigned long long *hits;
 };
 
-// Bottom-up probe: one wave per 1024-vertex sub-tile, three batches of
-// loads (visited and residency words; hints and their frontier bits;
-// degrees of the vertices the hints did not place).  Hint hits are written
-// into the wave's own visited / next words by the lanes that loaded them.
+constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
+
+// Bottom-up probe, one wave per 1024-vertex sub-tile:
+//   1. visited and residency words (lanes 0-31, one load each);
+//   2. hints of the unvisited vertices and their frontier bits; a hit
+//      places the vertex (it is at level L+1 whichever way the level runs);
+//   3. degrees of the rest.  Lists of 1..D records are scanned in full
+//      right here: the wave gathers them in LDS, then each lane takes one
+//      (list address, its records, their frontier bits: three rounds of
+//      loads per 64 lists).  Hits of both kinds are gathered per bitmap
+//      word in LDS and written by the lane that loaded the word.
+//   Counts: tile set 0 holds the short lists scanned (entries, records),
+//   tile set 1 the long unvisited lists (degree > D) left for the
+//   early-exit kernel, both split by residency.
+template <int REC>
 __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
   __shared__ unsigned long long s[CBS / 64][8];
+  __shared__ uint32_t s_v[CBS / 64][SUB];   // short lists of the wave: vertex
+  __shared__ uint8_t s_d[CBS / 64][SUB];    //   and degree
+  __shared__ uint32_t s_hit[CBS / 64][32];  // hit mask per bitmap word
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long lt = lanemask_lt();
   unsigned long long hits = 0;
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB, w0 = base >> 5;
     const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
     const uint32_t vw = owner ? a.vis[w0 + lane] : 0xFFFFFFFFu;
     const uint32_t rw = owner ? a.rbit[w0 + lane] : 0u;
+    if (lane < 32) s_hit[wv][lane] = 0;
     uint32_t un = 0, dev = 0;  // bit r: round r's vertex is unvisited / HBM-resident
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
@@ -335,6 +355,7 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
       if (in && !((uint32_t)__shfl((int)vw, src, 64) & bit)) un |= 1u << r;
       if ((uint32_t)__shfl((int)rw, src, 64) & bit) dev |= 1u << r;
     }
+    uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
     if (a.hint) {
       uint32_t h[ROUNDS];
 #pragma unroll
@@ -344,32 +365,57 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++)
         if (h[r] != NO_HINT && ((a.front[h[r] >> 5] >> (h[r] & 31)) & 1u)) f |= 1u << r;
-      uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++) {
         const unsigned long long b = __ballot((f >> r) & 1u);
         if (lane == 2 * r) m = (uint32_t)b;
         if (lane == 2 * r + 1) m = (uint32_t)(b >> 32);
       }
-      if (owner && m) {
-        a.vis[w0 + lane] = vw | m;
-        a.next[w0 + lane] |= m;
-        hits += __popc(m);
-      }
       un &= ~f;
     }
     uint32_t d[ROUNDS];
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
-    // sums: [0..3] short set (host n, dev n, host e, dev e), [4..7] long set
+    // counts, and the wave's short lists in LDS (in vertex order)
     unsigned long long x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t ns = 0;
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
       const uint32_t q = (dev >> r) & 1u;
+      const bool sh = d[r] != 0 && d[r] <= a.D;
+      const unsigned long long b = __ballot(sh);
+      if (sh) {
+        const uint32_t pos = ns + __popcll(b & lt);
+        s_v[wv][pos] = (uint32_t)(base + r * 64 + lane);
+        s_d[wv][pos] = (uint8_t)d[r];
+      }
+      ns += __popcll(b);
       if (d[r] == 0) continue;
-      const uint32_t k = d[r] <= a.D ? 0 : 4;
+      const uint32_t k = sh ? 0 : 4;
       x[k + q] += 1;
       x[k + 2 + q] += d[r];
+    }
+    __syncthreads();  // s_v / s_d / s_hit visible to the wave's lanes
+    for (uint32_t i = lane; i < ns; i += 64) {
+      const uint32_t v = s_v[wv][i], dl = s_d[wv][i];
+      const unsigned long long eb = a.eptr[v];
+      uint32_t w[SHORT_MAX];
+#pragma unroll
+      for (uint32_t k = 0; k < SHORT_MAX; k++)
+        w[k] = k < dl ? *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)k * REC)
+                      : 0xFFFFFFFFu;
+      bool hit = false;
+#pragma unroll
+      for (uint32_t k = 0; k < SHORT_MAX; k++)
+        if (w[k] != 0xFFFFFFFFu && ((a.front[w[k] >> 5] >> (w[k] & 31)) & 1u)) hit = true;
+      if (hit) atomicOr(&s_hit[wv][(v - base) >> 5], 1u << (v & 31));
+    }
+    __syncthreads();
+    hits += __popc(m);  // hint hits
+    if (lane < 32) m |= s_hit[wv][lane];
+    if (owner && m) {
+      a.vis[w0 + lane] = vw | m;
+      a.next[w0 + lane] |= m;
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) x[i] = wave_sum(x[i]);
@@ -428,7 +474,7 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   a.pres = st.d_pres.as<uint8_t>();
   a.P = st.P;
   a.n = g.n;
-  a.ntiles = (uint32_t)(g.n / TILE + 1);
+  a.ntiles = (uint32_t)std::max<uint64_t>(1, div_up(g.n, TILE));
   a.tcnt = w.tcnt[tset].as<uint32_t>();
   a.tdeg = w.tdeg[tset].as<unsigned long long>();
   a.tbc = w.tbc[tset].as<uint32_t>();
@@ -479,15 +525,19 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   p.vis = vis;
   p.next = next;
   p.n = g.n;
-  p.ntiles = (uint32_t)(g.n / TILE + 1);
+  p.ntiles = (uint32_t)std::max<uint64_t>(1, div_up(g.n, TILE));
   p.D = D;
   for (int k = 0; k < 2; k++) {
     p.tcnt[k] = w.tcnt[k].as<uint32_t>();
     p.tdeg[k] = w.tdeg[k].as<unsigned long long>();
   }
   p.hits = hits;
+  p.eptr = g.store.d_eptr.as<uint64_t>();
+  p.rec = (uint32_t)g.rec;
+  if (D > SHORT_MAX) fail(THV_EINVAL, "bottom-up full-scan degree above 16");
   const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
-  hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);
+  if (g.rec == 8) hipLaunchKernelGGL(k_bu_probe<8>, dim3(grid), dim3(CBS), 0, s, p);
+  else hipLaunchKernelGGL(k_bu_probe<4>, dim3(grid), dim3(CBS), 0, s, p);
   // tile bases and totals of both sets
   for (int k = 0; k < 2; k++) {
     CompactArgs a;

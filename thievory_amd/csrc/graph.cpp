@@ -217,6 +217,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   uint64_t used = 0;
   s.dev_edges = 0;
   s.Pdev = 0;
+  std::vector<uint32_t> chosen;  // partitions taken, in the order taken
   for (uint32_t p : order) {
     uint64_t e = pedges(p);
     if (e == 0) {  // nothing to store; count it as device resident
@@ -229,6 +230,37 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
     used += e * rec;
     s.dev_edges += e;
     s.Pdev++;
+    chosen.push_back(p);
+  }
+
+  // Short-list side area (bottom-up layout): the lists of at most
+  // SIDE_DEGREE records that are left in host partitions are copied to HBM
+  // as well, so a bottom-up level reads every short list from HBM.  They
+  // are few (the sparsest partitions are resident already) and charged to
+  // the budget: the last partitions taken make room if needed.
+  constexpr uint32_t SIDE_DEGREE = 16;
+  auto short_edges = [&](uint32_t p) {
+    uint64_t e = 0;
+    for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) {
+      const uint64_t d = g.h_off[v + 1] - g.h_off[v];
+      if (d >= 1 && d <= SIDE_DEGREE) e += d;
+    }
+    return e;
+  };
+  uint64_t side_edges = 0;
+  if (sparse_first && budget != UINT64_MAX) {
+    for (uint32_t p = 0; p < s.P; p++)
+      if (!s.pres[p]) side_edges += short_edges(p);
+    while (used + side_edges * rec > budget && !chosen.empty()) {
+      const uint32_t p = chosen.back();
+      chosen.pop_back();
+      s.pres[p] = 0;
+      used -= pedges(p) * rec;
+      s.dev_edges -= pedges(p);
+      s.Pdev--;
+      side_edges += short_edges(p);
+    }
+    if (used + side_edges * rec > budget) side_edges = 0;  // no room at all
   }
 
   // Slab: resident partitions packed in vertex order; contiguous runs of
@@ -257,22 +289,48 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
     p = q;
   }
 
+  // Side area: the short host lists, packed in vertex order.
+  s.side.alloc(side_edges * rec);
+  s.side_edges = side_edges;
+  s.dev_edges += side_edges;
+  if (side_edges) {
+    std::vector<char> stage(side_edges * rec);
+    uint64_t at = 0;
+    for (uint32_t p = 0; p < s.P; p++) {
+      if (s.pres[p]) continue;
+      for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) {
+        const uint64_t d = g.h_off[v + 1] - g.h_off[v];
+        if (d < 1 || d > SIDE_DEGREE) continue;
+        memcpy(stage.data() + at * rec, (const char *)g.h_edges.p + g.h_off[v] * rec, d * rec);
+        at += d;
+      }
+    }
+    HIPC(hipMemcpy(s.side.p, stage.data(), side_edges * rec, hipMemcpyHostToDevice));
+  }
+
   // Per-vertex first-record address, residency bit and partition id.
   std::vector<uint64_t> eptr(g.n);
   std::vector<uint32_t> rbit(bitmap_words(g.n), 0u), vpart(g.n);
   const uint64_t slab_base = (uint64_t)(uintptr_t)s.slab.p;
+  const uint64_t side_base = (uint64_t)(uintptr_t)s.side.p;
   const uint64_t host_base = (uint64_t)(uintptr_t)g.h_edges.dev;
   s.host_vertices = 0;
+  uint64_t side_at = 0;
   for (uint32_t p = 0; p < s.P; p++) {
     uint64_t pe0 = g.h_off[s.pv[p]];
     for (uint64_t v = s.pv[p]; v < s.pv[p + 1]; v++) {
       vpart[v] = p;
+      const uint64_t d = g.h_off[v + 1] - g.h_off[v];
       if (s.pres[p]) {
         eptr[v] = slab_base + (slab_off[p] + (g.h_off[v] - pe0)) * rec;
         rbit[v >> 5] |= 1u << (v & 31);
+      } else if (side_edges && d >= 1 && d <= SIDE_DEGREE) {
+        eptr[v] = side_base + side_at * rec;
+        rbit[v >> 5] |= 1u << (v & 31);
+        side_at += d;
       } else {
         eptr[v] = host_base + g.h_off[v] * rec;
-        s.host_vertices += g.h_off[v + 1] != g.h_off[v];
+        s.host_vertices += d != 0;
       }
     }
   }

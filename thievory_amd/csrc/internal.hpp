@@ -80,6 +80,8 @@ struct Store {
   std::vector<uint32_t> pv;          // P+1 first vertex of each partition
   std::vector<uint8_t> pres;         // P residency (1 = HBM)
   DevMem slab;                       // resident records
+  DevMem side;                       // short host lists copied to HBM
+  uint64_t side_edges = 0;           //   (bottom-up layout; in dev_edges)
   DevMem d_pv, d_pres, d_eptr;
   DevMem d_rbit;                     // n bits: vertex's edges are in HBM
   DevMem d_vpart;                    // n x u32: partition of each vertex
@@ -196,13 +198,10 @@ void workspace_ensure(Graph &g);
 uint64_t now_ns();
 
 // kernels (frontier.hip / expand.hip / algos.hip)
-// A_BFS_PULL: bottom-up BFS as an edge-balanced scan -- the queue holds
-// unvisited vertices; one of them joins the next frontier when any of its
-// edges leads into the current frontier.
 // A_HINT: parent-hint build -- key64[v] = max over v's edges (v, w) of
 // (deg[w] << 32 | ~w).
 enum Algo {
-  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_BFS_PULL = 5, A_HINT = 6
+  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_HINT = 6
 };
 
 struct ExpandArgs {
@@ -214,7 +213,6 @@ struct ExpandArgs {
   uint32_t *val;            // BFS/CC/SSSP values
   uint32_t *vis;            // BFS visited bitmap
   uint32_t *next;           // next frontier bitmap
-  const uint32_t *front;    // BFS pull: current frontier bitmap
   const uint32_t *deg;      // hint build: degrees
   unsigned long long *key64;  // hint build: per-vertex best key
   float *pr_src;            // PR push: delta; PR pull: contribution
