@@ -80,6 +80,11 @@ typedef struct {
   uint64_t bottom_up_host_bytes; /* part of host_bytes read bottom-up       */
   uint64_t hint_hits;            /* bottom-up BFS: vertices placed by their
                                     parent hint, without reading their list */
+  double probe_ms;               /* bottom-up BFS: time inside the probe
+                                    kernel (events), summed over its launches */
+  uint32_t probe_launches;
+  uint32_t pad2;
+  uint64_t probe_hbm_bytes;      /* algorithmic HBM bytes of those launches */
 } thv_stats;
 
 /* ---- graph lifetime --------------------------------------------------- */

@@ -200,6 +200,8 @@ def test_parent_hints(rmat16, rmat16u):
             assert np.array_equal(lv, O.bfs(og, s)), s
             if s == 1:
                 assert st.bottom_up_levels > 0 and st.hint_hits > 0
+                assert st.probe_launches >= st.bottom_up_levels
+                assert st.probe_hbm_bytes > 0 and st.probe_ms > 0
         g.set_option("parent_hints", 0)   # rebuilds the store without hints
         assert g.get_option("hint_bytes") == 0
         assert g.store_info()["device_edges"] > dev_with

@@ -50,10 +50,12 @@ class Stats(C.Structure):
                 ("edges_examined", C.c_uint64), ("bottom_up_levels", C.c_uint32),
                 ("pad", C.c_uint32), ("topdown_host_bytes", C.c_uint64),
                 ("topdown_hbm_bytes", C.c_uint64), ("bottom_up_ms", C.c_double),
-                ("bottom_up_host_bytes", C.c_uint64), ("hint_hits", C.c_uint64)]
+                ("bottom_up_host_bytes", C.c_uint64), ("hint_hits", C.c_uint64),
+                ("probe_ms", C.c_double), ("probe_launches", C.c_uint32),
+                ("pad2", C.c_uint32), ("probe_hbm_bytes", C.c_uint64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("pad")}
 
 
 class ServeCfg(C.Structure):

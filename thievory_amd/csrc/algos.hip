@@ -265,15 +265,34 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const bool host_share = g.store.host_vertices > 0;
   const double link = g.do_link_gbps * 1e9;
   const bool hints = host_share && g.store.d_hint.p != nullptr;
-  unsigned long long *d_hits = w.aux.as<unsigned long long>() + 4;
+  // probe counters: hint hits, unvisited, unvisited with a hint, words written
+  unsigned long long *d_pc = w.aux.as<unsigned long long>() + 4;
   // per-vertex costs of a bottom-up level over host lists: a full scan of
   // a short list costs its bytes plus SHORT_NS (a partly shared request),
   // an early-exit scan of a long list do_vertex_ns (measured on MI355X)
   constexpr double SHORT_NS = 0.6;
-  uint64_t visited_edges = 0, prev_nf = 0, seen = 0;
-  double expand_ms = 0, bottom_up_ms = 0;
-  uint32_t launches = 0;
-  if (hints) HIPC(hipMemsetAsync(d_hits, 0, 8, s));
+  uint64_t visited_edges = 0, prev_nf = 0;
+  double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0;
+  uint32_t launches = 0, probes = 0;
+  uint64_t probe_bytes = 0;  // the parts that do not come from d_pc
+  HIPC(hipMemsetAsync(d_pc, 0, 4 * 8, s));
+  auto probe = [&](const uint32_t *hint, uint32_t *next) {
+    launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint, D,
+                    c_short, c_long, d_pc, s, w.ev[2], w.ev[3]);
+    probes++;
+    // visited and residency words, tile tables of both sets
+    probe_bytes += 2 * 4 * words + 2 * 24 * ((g.n + TILE - 1) / TILE);
+  };
+  auto probe_done = [&]() {  // after the stream has drained past the probe
+    float pm = 0;
+    HIPC(hipEventElapsedTime(&pm, w.ev[2], w.ev[3]));
+    probe_ms += pm;
+    // short lists: an address each, their HBM records, and a frontier word
+    // per record (records read from host memory count as host bytes)
+    const LevelCtr &ps = w.h_ctr[3];
+    const uint64_t sn = ps.qn[0] + ps.qn[1], sr = ps.qw[0] + ps.qw[1];
+    probe_bytes += 8 * sn + rec * ps.qw[1] + 4 * sr;
+  };
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
@@ -284,7 +303,6 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     const LevelCtr h = w.h_ctr[L & 1];
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
     if (nf == 0) break;
-    seen += nf;
     visited_edges += mf;
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
     // probed: the bottom-up probe ran (tile set 0 now holds the short
@@ -305,11 +323,11 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       const double probe_cost = 2.0 * ((double)g.n * 6.0 / 5e12 + 15e-6);
       bu = false;
       if (td_cost > probe_cost) {
-        launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint,
-                        D, c_short, c_long, d_hits, s);
+        probe(hint, next);
         HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
-        if (trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_hits, 8, hipMemcpyDeviceToHost, s));
+        if (trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
+        probe_done();
         const LevelCtr &pl = w.h_ctr[2], &ps = w.h_ctr[3];
         bu_cost = (double)ps.qw[0] * rec / link + (double)ps.qn[0] * SHORT_NS * 1e-9 +
                   (double)pl.qn[0] * g.do_vertex_ns * 1e-9;
@@ -333,9 +351,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       launch_expand(g, A_BFS, a, s);
       HIPC(hipEventRecord(w.ev[1], s));
     } else {
-      if (!probed)
-        launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint,
-                        D, c_short, c_long, d_hits, s);
+      if (!probed) probe(hint, next);
       HIPC(hipEventRecord(w.ev[0], s));
       // the probe scanned the short lists already; the long set is the one
       // it counted
@@ -358,6 +374,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     }
     HIPC(hipStreamSynchronize(s));
+    if (bu && !probed) probe_done();
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
     LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
@@ -387,8 +404,6 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       st->iterations++;
       st->topdown_host_bytes += h.qw[0] * rec;
       st->topdown_hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
-      if (hinted)  // hint words and frontier probes of the unvisited vertices
-        st->hbm_bytes += 8 * (g.n - std::min<uint64_t>(g.n, seen)) + 8 * words;
       if (!bu) {
         st->edges_examined += mf;
         st->host_bytes += h.qw[0] * rec;
@@ -399,10 +414,11 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         st->edges_examined += rh + rd;
         st->host_bytes += rh * rec;
         st->bottom_up_host_bytes += rh * rec;
-        // records in HBM, one frontier-bit probe per record, 12 bytes per
-        // unvisited vertex, and the visited / frontier bitmaps
-        st->hbm_bytes += rd * rec + 4 * (rh + rd) +
-                         12 * (hs.qn[0] + hs.qn[1] + hb.qn[0] + hb.qn[1]) + 12 * words;
+        // early-exit part (the probe's bytes are added at the end): records
+        // in HBM, a frontier word per record, 12 bytes per long unvisited
+        // list, and the visited / frontier bitmaps
+        st->hbm_bytes += hb.rd[1] * rec + 4 * (hb.rd[0] + hb.rd[1]) +
+                         12 * (hb.qn[0] + hb.qn[1]) + 12 * words;
         st->host_partitions_touched += hb.touched[0] + hs.touched[0];
         st->bottom_up_levels++;
       }
@@ -414,9 +430,19 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     st->expand_ms = expand_ms;
     st->expand_launches = launches;
     st->bottom_up_ms = bottom_up_ms;
-    if (hints) {
-      HIPC(hipMemcpy(w.h_aux, d_hits, 8, hipMemcpyDeviceToHost));
-      st->hint_hits = w.h_aux[0];
+    if (probes) {
+      HIPC(hipMemcpy(w.h_aux, d_pc, 4 * 8, hipMemcpyDeviceToHost));
+      const unsigned long long *pc = w.h_aux;
+      st->hint_hits = pc[0];
+      st->probe_ms = probe_ms;
+      st->probe_launches = probes;
+      // per unvisited vertex: its hint word (when hints are on) and, when
+      // it has one, the frontier word it names; its degree unless a hint
+      // placed it; 12 bytes per bitmap word updated (visited written,
+      // next read and written)
+      st->probe_hbm_bytes = probe_bytes + (hints ? 4 * pc[1] + 4 * pc[2] : 0) +
+                            4 * (pc[1] - pc[0]) + 12 * pc[3];
+      st->hbm_bytes += st->probe_hbm_bytes;
     }
   }
 }

@@ -116,22 +116,29 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB;
-    SubTile t;
-    load_subtile(a, base, lane, t);
-    // BFS: every frontier vertex (degree 0 included) takes the level here,
-    // in vertex order, instead of by a scattered store when it was found
-    if (a.val) {
+    unsigned long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    // fast path: no word of the sub-tile selects anything
+    const uint64_t w0 = base >> 5;
+    const uint32_t fw = lane < 32 && ((w0 + lane) << 5) < a.n ? a.front[w0 + lane]
+                                                              : (a.invert ? 0xFFFFFFFFu : 0u);
+    if (__ballot(a.invert ? fw != 0xFFFFFFFFu : fw != 0u)) {
+      SubTile t;
+      load_subtile(a, base, lane, t);
+      // BFS: every frontier vertex (degree 0 included) takes the level
+      // here, in vertex order, instead of by a scattered store when found
+      if (a.val) {
 #pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++)
-        if ((t.raw >> r) & 1u) a.val[base + r * 64 + lane] = a.val_value;
+        for (uint32_t r = 0; r < ROUNDS; r++)
+          if ((t.raw >> r) & 1u) a.val[base + r * 64 + lane] = a.val_value;
+      }
+      uint32_t c[2];
+      unsigned long long e[2];
+      subtile_sums(t, c, e);
+      v0 = wave_sum(c[0]);
+      v1 = wave_sum(c[1]);
+      v2 = wave_sum(e[0]);
+      v3 = wave_sum(e[1]);
     }
-    uint32_t c[2];
-    unsigned long long e[2];
-    subtile_sums(t, c, e);
-    const unsigned long long v0 = wave_sum((unsigned long long)c[0]);
-    const unsigned long long v1 = wave_sum((unsigned long long)c[1]);
-    const unsigned long long v2 = wave_sum(e[0]);
-    const unsigned long long v3 = wave_sum(e[1]);
     if (lane == 0) {
       s[wv][0] = v0;
       s[wv][1] = v1;
@@ -154,8 +161,12 @@ constexpr int SNW = SBS / 64;
 
 // Exclusive tile bases.  Wave w owns the contiguous tile segment
 // [w * per, (w + 1) * per): a first sweep sums it (64 tiles per coalesced
-// load), the waves' sums are scanned in LDS, and a second sweep writes the
-// bases with a wave scan per 64 tiles.
+// load, the first KC loads kept in registers), the waves' sums are scanned
+// in LDS, and a second sweep writes the bases with a wave scan per 64
+// tiles.  KC = 12 keeps 768 tiles per wave (2^25 vertices in all) in
+// registers without spilling.
+constexpr uint32_t KC = 12;
+
 __global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
   __shared__ unsigned long long sh[SNW][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -163,9 +174,26 @@ __global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
   const uint32_t s0 = min(a.ntiles, wv * per), s1 = min(a.ntiles, s0 + per);
   const uint32_t *__restrict__ tcnt = a.tcnt;
   const unsigned long long *__restrict__ tdeg = a.tdeg;
+  uint32_t cc0[KC], cc1[KC];
+  unsigned long long ce0[KC], ce1[KC];
   unsigned long long x[4] = {0, 0, 0, 0};
-#pragma unroll 4
-  for (uint32_t t = s0 + lane; t < s1; t += 64) {
+#pragma unroll
+  for (uint32_t k = 0; k < KC; k++) {
+    const uint32_t t = s0 + k * 64 + lane;
+    const bool ok = t < s1;
+    cc0[k] = ok ? tcnt[2 * t] : 0u;
+    cc1[k] = ok ? tcnt[2 * t + 1] : 0u;
+    ce0[k] = ok ? tdeg[2 * t] : 0ull;
+    ce1[k] = ok ? tdeg[2 * t + 1] : 0ull;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < KC; k++) {
+    x[0] += cc0[k];
+    x[1] += cc1[k];
+    x[2] += ce0[k];
+    x[3] += ce1[k];
+  }
+  for (uint32_t t = s0 + KC * 64 + lane; t < s1; t += 64) {
     x[0] += tcnt[2 * t];
     x[1] += tcnt[2 * t + 1];
     x[2] += tdeg[2 * t];
@@ -183,25 +211,34 @@ __global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
 #pragma unroll
     for (int i = 0; i < 4; i++) run[i] += sh[w][i];
   }
-  for (uint32_t t0 = s0; t0 < s1; t0 += 64) {
-    const uint32_t t = t0 + lane;
+  // counts are below 2^32 (n is), edge sums need 64 bits
+  auto step = [&](uint32_t t, const unsigned long long (&v)[4]) {
     const bool ok = t < s1;
-    unsigned long long v[4];
-    v[0] = ok ? tcnt[2 * t] : 0u;
-    v[1] = ok ? tcnt[2 * t + 1] : 0u;
-    v[2] = ok ? tdeg[2 * t] : 0ull;
-    v[3] = ok ? tdeg[2 * t + 1] : 0ull;
-    unsigned long long inc[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) inc[i] = wave_incl_scan(v[i]);
+    const uint32_t i0 = wave_incl_scan((uint32_t)v[0]), i1 = wave_incl_scan((uint32_t)v[1]);
+    const unsigned long long i2 = wave_incl_scan(v[2]), i3 = wave_incl_scan(v[3]);
     if (ok) {
-      a.tbc[2 * t] = (uint32_t)(run[0] + inc[0] - v[0]);
-      a.tbc[2 * t + 1] = (uint32_t)(run[1] + inc[1] - v[1]);
-      a.tbe[2 * t] = run[2] + inc[2] - v[2];
-      a.tbe[2 * t + 1] = run[3] + inc[3] - v[3];
+      a.tbc[2 * t] = (uint32_t)run[0] + i0 - (uint32_t)v[0];
+      a.tbc[2 * t + 1] = (uint32_t)run[1] + i1 - (uint32_t)v[1];
+      a.tbe[2 * t] = run[2] + i2 - v[2];
+      a.tbe[2 * t + 1] = run[3] + i3 - v[3];
     }
+    run[0] += (uint32_t)__shfl((int)i0, 63, 64);
+    run[1] += (uint32_t)__shfl((int)i1, 63, 64);
+    run[2] += __shfl(i2, 63, 64);
+    run[3] += __shfl(i3, 63, 64);
+  };
 #pragma unroll
-    for (int i = 0; i < 4; i++) run[i] += __shfl(inc[i], 63, 64);
+  for (uint32_t k = 0; k < KC; k++) {
+    if (s0 + k * 64 >= s1) break;
+    const unsigned long long v[4] = {cc0[k], cc1[k], ce0[k], ce1[k]};
+    step(s0 + k * 64 + lane, v);
+  }
+  for (uint32_t k = KC; s0 + k * 64 < s1; k++) {
+    const uint32_t t = s0 + k * 64 + lane;
+    const bool ok = t < s1;
+    const unsigned long long v[4] = {ok ? tcnt[2 * t] : 0u, ok ? tcnt[2 * t + 1] : 0u,
+                                     ok ? tdeg[2 * t] : 0ull, ok ? tdeg[2 * t + 1] : 0ull};
+    step(t, v);
   }
   // the last wave's running sums end at the grand totals
   if (wv == SNW - 1 && lane == 0) {
@@ -314,7 +351,7 @@ struct ProbeArgs {
   uint32_t ntiles, D, rec;
   uint32_t *tcnt[2];
   unsigned long long *tdeg[2];
-  unsigned long long *hits;
+  unsigned long long *cnt;  // [0] hint hits, [1] unvisited, [2] with a hint, [3] words written
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
@@ -339,7 +376,7 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
   __shared__ uint32_t s_hit[CBS / 64][32];  // hit mask per bitmap word
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = lanemask_lt();
-  unsigned long long hits = 0;
+  unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0;
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB, w0 = base >> 5;
     const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
@@ -355,12 +392,15 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
       if (in && !((uint32_t)__shfl((int)vw, src, 64) & bit)) un |= 1u << r;
       if ((uint32_t)__shfl((int)rw, src, 64) & bit) dev |= 1u << r;
     }
+    nun += __popc(un);
     uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
     if (a.hint) {
       uint32_t h[ROUNDS];
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++)
         h[r] = ((un >> r) & 1u) ? a.hint[base + r * 64 + lane] : NO_HINT;
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++) nhint += h[r] != NO_HINT;
       uint32_t f = 0;
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++)
@@ -416,6 +456,7 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     if (owner && m) {
       a.vis[w0 + lane] = vw | m;
       a.next[w0 + lane] |= m;
+      nwords++;
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) x[i] = wave_sum(x[i]);
@@ -433,14 +474,18 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     }
     __syncthreads();
   }
-  // one atomic per block (the grid is a few blocks per CU)
-  hits = wave_sum(hits);
-  if (lane == 0) s[wv][0] = hits;
+  // one atomic per counter and block (the grid is a few blocks per CU)
+  const unsigned long long c4[4] = {wave_sum(hits), wave_sum(nun), wave_sum(nhint),
+                                    wave_sum(nwords)};
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) s[wv][i] = c4[i];
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 4) {
     unsigned long long t = 0;
-    for (int w = 0; w < CBS / 64; w++) t += s[w][0];
-    if (t) atomicAdd(a.hits, t);
+    for (int w = 0; w < CBS / 64; w++) t += s[w][threadIdx.x];
+    if (t) atomicAdd(&a.cnt[threadIdx.x], t);
   }
 }
 
@@ -515,7 +560,8 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
 
 void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis,
                      uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
-                     LevelCtr *cl, unsigned long long *hits, hipStream_t s) {
+                     LevelCtr *cl, unsigned long long *cnt, hipStream_t s, hipEvent_t t0,
+                     hipEvent_t t1) {
   ProbeArgs p;
   memset(&p, 0, sizeof p);
   p.front = front;
@@ -531,13 +577,15 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
     p.tcnt[k] = w.tcnt[k].as<uint32_t>();
     p.tdeg[k] = w.tdeg[k].as<unsigned long long>();
   }
-  p.hits = hits;
+  p.cnt = cnt;
   p.eptr = g.store.d_eptr.as<uint64_t>();
   p.rec = (uint32_t)g.rec;
   if (D > SHORT_MAX) fail(THV_EINVAL, "bottom-up full-scan degree above 16");
   const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
+  if (t0) HIPC(hipEventRecord(t0, s));
   if (g.rec == 8) hipLaunchKernelGGL(k_bu_probe<8>, dim3(grid), dim3(CBS), 0, s, p);
   else hipLaunchKernelGGL(k_bu_probe<4>, dim3(grid), dim3(CBS), 0, s, p);
+  if (t1) HIPC(hipEventRecord(t1, s));
   // tile bases and totals of both sets
   for (int k = 0; k < 2; k++) {
     CompactArgs a;

@@ -143,7 +143,7 @@ struct Workspace {
   DevMem aux;                 // misc device counters (8 x u64)
   LevelCtr *h_ctr = nullptr;  // pinned readback
   unsigned long long *h_aux = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // [2], [3]: probe
   ~Workspace();
 };
 
@@ -234,14 +234,18 @@ void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr
                     uint32_t val_value = 0, uint32_t dmin = 1,
                     uint32_t dmax = 0xFFFFFFFFu, int tset = 0);
 // Bottom-up probe, one sweep over the vertices: every unvisited vertex
-// whose parent hint (hint may be null) is in front joins vis and next
-// (hits summed into *hits); the other unvisited vertices are counted as
-// two sets for WRITE_ONLY compactions of the inverted vis bitmap: degree
-// 1..D in tile set 0 with totals in *cs, degree > D in tile set 1 with
-// totals in *cl.
+// whose parent hint (hint may be null) is in front joins vis and next, and
+// so does every unvisited vertex of degree 1..D with a frontier vertex in
+// its list (scanned in full).  Tile set 0 / *cs count those short lists
+// (entries, records read); tile set 1 / *cl the long unvisited lists
+// (degree > D) for a WRITE_ONLY compaction of the inverted vis bitmap.
+// cnt[0..3] accumulate hint hits, unvisited vertices, unvisited vertices
+// with a hint, and bitmap words written.  Events t0 / t1 (may be null)
+// bracket the probe kernel alone.
 void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vis,
                      uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
-                     LevelCtr *cl, unsigned long long *hits, hipStream_t s);
+                     LevelCtr *cl, unsigned long long *cnt, hipStream_t s,
+                     hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 
 struct BottomUpArgs {
   const uint32_t *qv[2];
