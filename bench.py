@@ -111,17 +111,29 @@ def serving_bench(T, args):
     b64 = rng.integers(0, g.n, 64).astype(np.uint32)
     deg = np.diff(g.offsets()).astype(np.int64)
     out["sources_isolated"] = int((deg[b64] == 0).sum())
-    for algo, fn in (("bfs", g.bfs), ("sssp", g.sssp)):
+    def plain_qps(fn):
         fn(int(b64[0]), out=False)
         t = time.perf_counter()
         for s in b64:
             fn(int(s), out=False)
-        out[f"plain_{algo}_qps"] = round(64 / (time.perf_counter() - t), 2)
+        return round(64 / (time.perf_counter() - t), 2)
+
+    # plain BFS two ways: the reference's one-query-per-launch top-down
+    # kernel (the serving target's baseline), and this engine's own
+    # direction-optimising single query
+    g.set_option("direction", 0)
+    out["plain_bfs_topdown_qps"] = plain_qps(g.bfs)
+    g.set_option("direction", 1)
+    out["plain_bfs_qps"] = plain_qps(g.bfs)
+    out["plain_sssp_qps"] = plain_qps(g.sssp)
+    for algo in ("bfs", "sssp"):
         r = g.batch(algo, b64)
         out[f"batch64_{algo}_qps"] = round(64 / (r["stats"].ms / 1e3), 2)
         out[f"batch64_{algo}_levels"] = r["stats"].iterations
         out[f"batch64_{algo}_vs_plain"] = round(out[f"batch64_{algo}_qps"] /
                                                 out[f"plain_{algo}_qps"], 2)
+    out["batch64_bfs_vs_plain_topdown"] = round(out["batch64_bfs_qps"] /
+                                                out["plain_bfs_topdown_qps"], 2)
     log(f"plain/batch: {out}")
     # BFS-only stream: saturation, then Poisson at 70% of it
     nb = args.serve_queries // 2
@@ -131,6 +143,8 @@ def serving_bench(T, args):
     out["serve_bfs_saturated"] = sat
     out["serve_bfs_poisson70"] = pois
     out["serve_bfs_speedup_vs_plain"] = round(sat["qps"] / out["plain_bfs_qps"], 2)
+    out["serve_bfs_speedup_vs_plain_topdown"] = round(sat["qps"] /
+                                                      out["plain_bfs_topdown_qps"], 2)
     log(f"bfs stream: {sat} / {pois}")
     # the mixed 50/50 BFS+SSSP stream of the configuration
     qm = [("bfs" if rng.random() < 0.5 else "sssp", int(s))
