@@ -267,10 +267,14 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const bool hints = host_share && g.store.d_hint.p != nullptr;
   // probe counters: hint hits, unvisited, unvisited with a hint, words written
   unsigned long long *d_pc = w.aux.as<unsigned long long>() + 4;
-  // per-vertex costs of a bottom-up level over host lists: a full scan of
-  // a short list costs its bytes plus SHORT_NS (a partly shared request),
-  // an early-exit scan of a long list do_vertex_ns (measured on MI355X)
-  constexpr double SHORT_NS = 0.6;
+  // Cost terms of the split-store rule, measured on MI355X at R-MAT 26:
+  // a short host list scanned in full costs its bytes plus SHORT_NS (a
+  // partly shared request), a long one do_vertex_ns (one early-exit read);
+  // the HBM side of a top-down level costs TD_NS_ENTRY per queued vertex
+  // (compaction write) and TD_NS_EDGE per edge (expansion); a probe sweep
+  // PROBE_NS_VERTEX per vertex plus PROBE_FIXED (launches and readback).
+  constexpr double SHORT_NS = 0.6, TD_NS_ENTRY = 0.053, TD_NS_EDGE = 0.019,
+                   PROBE_NS_VERTEX = 0.0045, PROBE_FIXED = 30e-6;
   uint64_t visited_edges = 0, prev_nf = 0;
   double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0;
   uint32_t launches = 0, probes = 0;
@@ -313,14 +317,15 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     const uint32_t *hint = hints ? g.store.d_hint.as<uint32_t>() : nullptr;
     uint32_t *next = w.front[cur ^ 1].as<uint32_t>();
     if (host_share) {
-      // top-down streams the frontier's host records over the link; the
-      // hybrid bottom-up level streams the short host lists in full and
-      // pays one small zero-copy read per long unvisited host list.  The
-      // probe (hints + counts of the unvisited set, about 6 HBM bytes per
-      // vertex and two readbacks) runs only when the frontier's host
-      // records cost more.
-      td_cost = (double)h.qw[0] * rec / link;
-      const double probe_cost = 2.0 * ((double)g.n * 6.0 / 5e12 + 15e-6);
+      // top-down streams the frontier's host records over the link and
+      // writes and expands its queue in HBM; the hybrid bottom-up level
+      // streams the short host lists in full and pays one small zero-copy
+      // read per long unvisited host list.  The probe (hints, short lists,
+      // counts of the unvisited set) runs only when top-down costs more
+      // than the probe sweep itself.
+      td_cost = (double)h.qw[0] * rec / link +
+                ((double)nf * TD_NS_ENTRY + (double)mf * TD_NS_EDGE) * 1e-9;
+      const double probe_cost = (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED;
       bu = false;
       if (td_cost > probe_cost) {
         probe(hint, next);
@@ -456,7 +461,12 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   stats_begin(st);
   launch_fill_u32(w.val.as<uint32_t>(), (uint32_t)(g.n + 1), g.n, s);
   launch_fill_u32(w.val.as<uint32_t>() + src, 1u, 1, s);
-  HIPC(hipMemsetAsync(w.vis.p, 0, words * 4, s));
+  // on a symmetric graph a degree-0 vertex has no in-edges either: it
+  // starts visited (its level stays n+1), so bottom-up levels never look at it
+  if (g.sym && g.dir_opt)
+    HIPC(hipMemcpyAsync(w.vis.p, g.d_iso.p, words * 4, hipMemcpyDeviceToDevice, s));
+  else
+    HIPC(hipMemsetAsync(w.vis.p, 0, words * 4, s));
   HIPC(hipMemsetAsync(w.front[0].p, 0, words * 4, s));
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.vis.as<uint32_t>(), src);
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);

@@ -352,6 +352,8 @@ struct ProbeArgs {
   uint32_t *tcnt[2];
   unsigned long long *tdeg[2];
   unsigned long long *cnt;  // [0] hint hits, [1] unvisited, [2] with a hint, [3] words written
+  uint32_t *slist;          // short lists: wave (tile, w) writes from slot tile*TILE + w*SUB
+  uint32_t *scnt;           //   and their count at [tile * 4 + w]
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
@@ -359,21 +361,16 @@ constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
 // Bottom-up probe, one wave per 1024-vertex sub-tile:
 //   1. visited and residency words (lanes 0-31, one load each);
 //   2. hints of the unvisited vertices and their frontier bits; a hit
-//      places the vertex (it is at level L+1 whichever way the level runs);
-//   3. degrees of the rest.  Lists of 1..D records are scanned in full
-//      right here: the wave gathers them in LDS, then each lane takes one
-//      (list address, its records, their frontier bits: three rounds of
-//      loads per 64 lists).  Hits of both kinds are gathered per bitmap
-//      word in LDS and written by the lane that loaded the word.
-//   Counts: tile set 0 holds the short lists scanned (entries, records),
-//   tile set 1 the long unvisited lists (degree > D) left for the
-//   early-exit kernel, both split by residency.
-template <int REC>
+//      places the vertex (it is at level L+1 whichever way the level runs)
+//      and is written into the wave's own visited / next words by the lane
+//      that loaded them;
+//   3. degrees of the rest.  Lists of 1..D records go to the wave's slots
+//      of the short-list array, in vertex order, for k_bu_short.
+//   Counts: tile set 0 holds the short lists (entries, records), tile set
+//   1 the long unvisited lists (degree > D) left for the early-exit kernel,
+//   both split by residency.
 __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
   __shared__ unsigned long long s[CBS / 64][8];
-  __shared__ uint32_t s_v[CBS / 64][SUB];   // short lists of the wave: vertex
-  __shared__ uint8_t s_d[CBS / 64][SUB];    //   and degree
-  __shared__ uint32_t s_hit[CBS / 64][32];  // hit mask per bitmap word
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = lanemask_lt();
   unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0;
@@ -382,7 +379,6 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
     const uint32_t vw = owner ? a.vis[w0 + lane] : 0xFFFFFFFFu;
     const uint32_t rw = owner ? a.rbit[w0 + lane] : 0u;
-    if (lane < 32) s_hit[wv][lane] = 0;
     uint32_t un = 0, dev = 0;  // bit r: round r's vertex is unvisited / HBM-resident
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
@@ -416,7 +412,7 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     uint32_t d[ROUNDS];
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
-    // counts, and the wave's short lists in LDS (in vertex order)
+    // counts, and the wave's short lists (in vertex order)
     unsigned long long x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t ns = 0;
 #pragma unroll
@@ -424,35 +420,15 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
       const uint32_t q = (dev >> r) & 1u;
       const bool sh = d[r] != 0 && d[r] <= a.D;
       const unsigned long long b = __ballot(sh);
-      if (sh) {
-        const uint32_t pos = ns + __popcll(b & lt);
-        s_v[wv][pos] = (uint32_t)(base + r * 64 + lane);
-        s_d[wv][pos] = (uint8_t)d[r];
-      }
+      if (sh) a.slist[base + ns + __popcll(b & lt)] = (uint32_t)(base + r * 64 + lane);
       ns += __popcll(b);
       if (d[r] == 0) continue;
       const uint32_t k = sh ? 0 : 4;
       x[k + q] += 1;
       x[k + 2 + q] += d[r];
     }
-    __syncthreads();  // s_v / s_d / s_hit visible to the wave's lanes
-    for (uint32_t i = lane; i < ns; i += 64) {
-      const uint32_t v = s_v[wv][i], dl = s_d[wv][i];
-      const unsigned long long eb = a.eptr[v];
-      uint32_t w[SHORT_MAX];
-#pragma unroll
-      for (uint32_t k = 0; k < SHORT_MAX; k++)
-        w[k] = k < dl ? *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)k * REC)
-                      : 0xFFFFFFFFu;
-      bool hit = false;
-#pragma unroll
-      for (uint32_t k = 0; k < SHORT_MAX; k++)
-        if (w[k] != 0xFFFFFFFFu && ((a.front[w[k] >> 5] >> (w[k] & 31)) & 1u)) hit = true;
-      if (hit) atomicOr(&s_hit[wv][(v - base) >> 5], 1u << (v & 31));
-    }
-    __syncthreads();
+    if (lane == 0) a.scnt[tile * (CBS / 64) + wv] = ns;
     hits += __popc(m);  // hint hits
-    if (lane < 32) m |= s_hit[wv][lane];
     if (owner && m) {
       a.vis[w0 + lane] = vw | m;
       a.next[w0 + lane] |= m;
@@ -486,6 +462,59 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     unsigned long long t = 0;
     for (int w = 0; w < CBS / 64; w++) t += s[w][threadIdx.x];
     if (t) atomicAdd(&a.cnt[threadIdx.x], t);
+  }
+}
+
+struct ShortArgs {
+  const uint32_t *slist, *scnt, *deg, *front;
+  const uint64_t *eptr;
+  uint32_t *vis, *next;
+  uint32_t nregions;
+};
+
+// The short lists the probe collected: one wave per region (a wave's slots
+// of one probe sub-tile), one list per lane -- its address, its records and
+// their frontier words in three rounds of loads.  A list with a frontier
+// vertex places its vertex.
+template <int REC>
+__global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (CBS / 64);
+  for (uint32_t rg = blockIdx.x * (CBS / 64) + (threadIdx.x >> 6); rg < a.nregions;
+       rg += nwaves) {
+    const uint32_t cnt = a.scnt[rg];
+    const uint64_t base = (uint64_t)rg * SUB;
+    for (uint32_t i = lane; i < cnt; i += 64) {
+      const uint32_t v = a.slist[base + i];
+      const uint32_t dl = a.deg[v];  // 1..16
+      const unsigned long long eb = a.eptr[v];
+      // the first EAGER records load unconditionally (the index clamped to
+      // the list), so they are all in flight together; most lists are
+      // that short
+      constexpr uint32_t EAGER = 4;
+      uint32_t w[SHORT_MAX];
+#pragma unroll
+      for (uint32_t k = 0; k < EAGER; k++)
+        w[k] = *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)min(k, dl - 1) * REC);
+#pragma unroll
+      for (uint32_t k = EAGER; k < SHORT_MAX; k++)
+        w[k] = k < dl ? *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)k * REC)
+                      : 0xFFFFFFFFu;
+      uint32_t fw[EAGER];
+#pragma unroll
+      for (uint32_t k = 0; k < EAGER; k++) fw[k] = a.front[w[k] >> 5];
+      bool hit = false;
+#pragma unroll
+      for (uint32_t k = 0; k < EAGER; k++)
+        if ((fw[k] >> (w[k] & 31)) & 1u) hit = true;
+#pragma unroll
+      for (uint32_t k = EAGER; k < SHORT_MAX; k++)
+        if (w[k] != 0xFFFFFFFFu && ((a.front[w[k] >> 5] >> (w[k] & 31)) & 1u)) hit = true;
+      if (hit) {
+        atomicOr(&a.vis[v >> 5], 1u << (v & 31));
+        atomicOr(&a.next[v >> 5], 1u << (v & 31));
+      }
+    }
   }
 }
 
@@ -581,10 +610,24 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   p.eptr = g.store.d_eptr.as<uint64_t>();
   p.rec = (uint32_t)g.rec;
   if (D > SHORT_MAX) fail(THV_EINVAL, "bottom-up full-scan degree above 16");
+  p.slist = w.slist.as<uint32_t>();
+  p.scnt = w.scnt.as<uint32_t>();
   const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
   if (t0) HIPC(hipEventRecord(t0, s));
-  if (g.rec == 8) hipLaunchKernelGGL(k_bu_probe<8>, dim3(grid), dim3(CBS), 0, s, p);
-  else hipLaunchKernelGGL(k_bu_probe<4>, dim3(grid), dim3(CBS), 0, s, p);
+  hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);
+  if (D > 0) {
+    ShortArgs sa;
+    sa.slist = p.slist;
+    sa.scnt = p.scnt;
+    sa.deg = p.deg;
+    sa.front = front;
+    sa.eptr = p.eptr;
+    sa.vis = vis;
+    sa.next = next;
+    sa.nregions = p.ntiles * (CBS / 64);
+    if (g.rec == 8) hipLaunchKernelGGL(k_bu_short<8>, dim3(grid), dim3(CBS), 0, s, sa);
+    else hipLaunchKernelGGL(k_bu_short<4>, dim3(grid), dim3(CBS), 0, s, sa);
+  }
   if (t1) HIPC(hipEventRecord(t1, s));
   // tile bases and totals of both sets
   for (int k = 0; k < 2; k++) {

@@ -106,16 +106,19 @@ void graph_init_device(Graph &g) {
   HIPC(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
 }
 
-// Device degree tables + default store (everything in HBM).
+// Device degree tables: degrees and the degree-0 bitmap.
 void graph_build_tables(Graph &g) {
-  std::vector<uint32_t> deg(g.n);
+  std::vector<uint32_t> deg(g.n), iso(bitmap_words(g.n), 0u);
   for (uint64_t v = 0; v < g.n; v++) {
     uint64_t d = g.h_off[v + 1] - g.h_off[v];
     if (d > 0xFFFFFFFFull) fail(THV_ENOTSUP, "vertex degree exceeds 2^32-1");
     deg[v] = (uint32_t)d;
+    if (!d) iso[v >> 5] |= 1u << (v & 31);
   }
   g.d_deg.alloc(g.n * 4);
   if (g.n) HIPC(hipMemcpy(g.d_deg.p, deg.data(), g.n * 4, hipMemcpyHostToDevice));
+  g.d_iso.alloc(iso.size() * 4);
+  HIPC(hipMemcpy(g.d_iso.p, iso.data(), iso.size() * 4, hipMemcpyHostToDevice));
 }
 
 // Default store, built on first use when the caller configured none: like
@@ -356,6 +359,10 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
     P = g.store.P;
   }
   const uint64_t tiles = g.n / TILE + 1;
+  if (slist.bytes < tiles * TILE * 4) {
+    slist.alloc(tiles * TILE * 4);
+    scnt.alloc(tiles * 4 * 4);
+  }
   for (int k = 0; k < 2; k++) {
     if (tcnt[k].bytes < tiles * 2 * 4) {
       tcnt[k].alloc(tiles * 2 * 4);

@@ -130,6 +130,7 @@ struct QueueSet {
   DevMem tcnt[2], tdeg[2];    // active vertices / edges
   DevMem tbc[2], tbe[2];      // exclusive bases
   DevMem ptouch;              // per partition: stamp of the last touching call
+  DevMem slist, scnt;         // bottom-up probe: short lists per wave region
   void ensure(const Graph &g, uint32_t C);
 };
 
@@ -163,6 +164,7 @@ struct Graph {
   std::vector<uint64_t> h_off;  // n+1
   HostPinned h_edges;         // m records, mapped
   DevMem d_deg;               // n u32
+  DevMem d_iso;               // bitmap: degree 0 (never reached on a symmetric graph)
   DevMem d_outdeg;            // n u32 (CSC only)
   Store store;
   Workspace ws;
