@@ -86,16 +86,32 @@ __global__ void __launch_bounds__(MBS) k_ms_bfs(MsArgs a) {
         dst[j] = *reinterpret_cast<const uint32_t *>(s_base[k] + (ci.e0 + l) * REC);
       }
     }
+    // phases over all EPT edges, each issuing its memory operations before
+    // any result is used: the seen masks (an absent edge reads entry 0 and
+    // is ignored), the seen atomics, the next-frontier atomics
+    unsigned long long sw[EPT], add[EPT], gain[EPT], on[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) sw[j] = a.seen[dst[j] != 0xFFFFFFFFu ? dst[j] : 0];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) add[j] = dst[j] != 0xFFFFFFFFu ? s_m[ent[j]] & ~sw[j] : 0ull;
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      sw[j] = ~0ull;
+      if (add[j]) sw[j] = atomicOr(&a.seen[dst[j]], add[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; j++) gain[j] = add[j] & ~sw[j];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      on[j] = ~0ull;
+      if (gain[j]) on[j] = atomicOr(&a.N[dst[j]], gain[j]);
+    }
 #pragma unroll
     for (int j = 0; j < EPT; j++) {
       const uint32_t w = dst[j];
-      if (w == 0xFFFFFFFFu) continue;
-      const unsigned long long m = s_m[ent[j]];
-      const unsigned long long add = m & ~a.seen[w];
-      if (!add) continue;
-      const unsigned long long gained = add & ~atomicOr(&a.seen[w], add);
+      const unsigned long long gained = gain[j];
       if (!gained) continue;
-      if (atomicOr(&a.N[w], gained) == 0ull) atomicOr(&a.nb[w >> 5], 1u << (w & 31));
+      if (on[j] == 0ull) atomicOr(&a.nb[w >> 5], 1u << (w & 31));
       atomicOr(&s_lm, gained);
       unsigned long long g = gained;
       while (g) {

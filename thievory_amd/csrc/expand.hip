@@ -109,33 +109,70 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       __syncthreads();
     }
 
+    // Relaxation in phases, each issuing the memory operations of all EPT
+    // edges before any result is used: the destination-state words (an
+    // absent edge reads word 0 and is ignored), then the returning atomics
+    // of the edges that need one, then the next-frontier bits.
+    if (ALGO == A_BFS) {
+      uint32_t vw[EPT], old[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; j++) {
-      const uint32_t w = dst[j];
-      if (w == 0xFFFFFFFFu) continue;
-      const uint32_t wd = w >> 5, bit = 1u << (w & 31);
-      if (ALGO == A_BFS) {
-        if (!(a.vis[wd] & bit)) {
-          // the winner only marks the next frontier; the next level's
-          // compaction writes the level of every frontier vertex
-          const uint32_t old = atomicOr(&a.vis[wd], bit);
-          if (!(old & bit)) atomicOr(&a.next[wd], bit);
-        }
-      } else if (ALGO == A_CC || ALGO == A_SSSP) {
+      for (int j = 0; j < EPT; j++) vw[j] = a.vis[dst[j] != 0xFFFFFFFFu ? dst[j] >> 5 : 0];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const uint32_t w = dst[j], bit = 1u << (w & 31);
+        old[j] = 0xFFFFFFFFu;
+        if (w != 0xFFFFFFFFu && !(vw[j] & bit)) old[j] = atomicOr(&a.vis[w >> 5], bit);
+      }
+      // the winner only marks the next frontier; the next level's
+      // compaction writes the level of every frontier vertex
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const uint32_t w = dst[j], bit = 1u << (w & 31);
+        if (!(old[j] & bit)) atomicOr(&a.next[w >> 5], bit);
+      }
+    } else if (ALGO == A_CC || ALGO == A_SSSP) {
+      uint32_t cur[EPT], old[EPT], nv[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) cur[j] = a.val[dst[j] != 0xFFFFFFFFu ? dst[j] : 0];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const uint32_t w = dst[j];
         const unsigned long long nv64 =
             (unsigned long long)s_u[ent[j]] + (ALGO == A_SSSP ? wt[j] : 0u);
-        if (nv64 < a.val[w]) {
-          const uint32_t nv = (uint32_t)nv64;
-          const uint32_t old = atomicMin(&a.val[w], nv);
-          if (nv < old && !(a.next[wd] & bit)) atomicOr(&a.next[wd], bit);
-        }
-      } else if (ALGO == A_HINT) {
-        atomicMax(&s_base[ent[j]], ((unsigned long long)a.deg[w] << 32) | (0xFFFFFFFFu - w));
-      } else if (ALGO == A_PR_PUSH) {
-        unsafeAtomicAdd(&a.pr_dst[w], s_f[ent[j]]);
-      } else {  // A_PR_PULL: w is an in-neighbour of the entry's vertex
-        atomicAdd(&s_f[ent[j]], a.pr_src[w]);
+        nv[j] = (uint32_t)nv64;
+        old[j] = 0;
+        if (w != 0xFFFFFFFFu && nv64 < cur[j]) old[j] = atomicMin(&a.val[w], nv[j]);
       }
+      uint32_t nw[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        nw[j] = 0xFFFFFFFFu;
+        if (nv[j] < old[j]) nw[j] = a.next[dst[j] >> 5];
+      }
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const uint32_t w = dst[j], bit = 1u << (w & 31);
+        if (!(nw[j] & bit)) atomicOr(&a.next[w >> 5], bit);
+      }
+    } else if (ALGO == A_PR_PUSH) {
+#pragma unroll
+      for (int j = 0; j < EPT; j++)
+        if (dst[j] != 0xFFFFFFFFu) unsafeAtomicAdd(&a.pr_dst[dst[j]], s_f[ent[j]]);
+    } else if (ALGO == A_PR_PULL) {  // dst[j] is an in-neighbour of the entry's vertex
+      float c[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) c[j] = a.pr_src[dst[j] != 0xFFFFFFFFu ? dst[j] : 0];
+#pragma unroll
+      for (int j = 0; j < EPT; j++)
+        if (dst[j] != 0xFFFFFFFFu) atomicAdd(&s_f[ent[j]], c[j]);
+    } else if (ALGO == A_HINT) {
+      uint32_t dw[EPT];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) dw[j] = a.deg[dst[j] != 0xFFFFFFFFu ? dst[j] : 0];
+#pragma unroll
+      for (int j = 0; j < EPT; j++)
+        if (dst[j] != 0xFFFFFFFFu)
+          atomicMax(&s_base[ent[j]], ((unsigned long long)dw[j] << 32) | (0xFFFFFFFFu - dst[j]));
     }
     if (ALGO == A_PR_PULL) {
       __syncthreads();

@@ -397,10 +397,15 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
         h[r] = ((un >> r) & 1u) ? a.hint[base + r * 64 + lane] : NO_HINT;
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++) nhint += h[r] != NO_HINT;
+      // the frontier words the hints name, loaded together (a vertex
+      // without a hint reads word 0 and is ignored)
+      uint32_t fw[ROUNDS];
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++) fw[r] = a.front[h[r] != NO_HINT ? h[r] >> 5 : 0];
       uint32_t f = 0;
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++)
-        if (h[r] != NO_HINT && ((a.front[h[r] >> 5] >> (h[r] & 31)) & 1u)) f |= 1u << r;
+        if (h[r] != NO_HINT && ((fw[r] >> (h[r] & 31)) & 1u)) f |= 1u << r;
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++) {
         const unsigned long long b = __ballot((f >> r) & 1u);
@@ -500,16 +505,15 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
       for (uint32_t k = EAGER; k < SHORT_MAX; k++)
         w[k] = k < dl ? *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)k * REC)
                       : 0xFFFFFFFFu;
-      uint32_t fw[EAGER];
+      // their frontier words, loaded together (an absent record reads word
+      // 0 and is ignored)
+      uint32_t fw[SHORT_MAX];
 #pragma unroll
-      for (uint32_t k = 0; k < EAGER; k++) fw[k] = a.front[w[k] >> 5];
+      for (uint32_t k = 0; k < SHORT_MAX; k++) fw[k] = a.front[w[k] != 0xFFFFFFFFu ? w[k] >> 5 : 0];
       bool hit = false;
 #pragma unroll
-      for (uint32_t k = 0; k < EAGER; k++)
-        if ((fw[k] >> (w[k] & 31)) & 1u) hit = true;
-#pragma unroll
-      for (uint32_t k = EAGER; k < SHORT_MAX; k++)
-        if (w[k] != 0xFFFFFFFFu && ((a.front[w[k] >> 5] >> (w[k] & 31)) & 1u)) hit = true;
+      for (uint32_t k = 0; k < SHORT_MAX; k++)
+        if (w[k] != 0xFFFFFFFFu && ((fw[k] >> (w[k] & 31)) & 1u)) hit = true;
       if (hit) {
         atomicOr(&a.vis[v >> 5], 1u << (v & 31));
         atomicOr(&a.next[v >> 5], 1u << (v & 31));
