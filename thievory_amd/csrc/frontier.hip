@@ -8,7 +8,8 @@
 // host round trip, over fixed tiles of 4096 vertices:
 //   k_tcount  one block per tile, one wave per 1024-vertex sub-tile: active
 //             vertices (degree > 0) and their edges, split by residency
-//   k_tscan   one block: exclusive tile bases per queue, totals
+//   k_sscan   one block: exclusive bases of the segments of 64 tiles (the
+//             count kernels add their tiles' sums into them), totals
 //   k_twrite  one block per tile with work: ordered queue entries (vertex
 //             id, first edge in the level's work space), the chunk-start
 //             table the expansion kernel indexes by chunk, and a stamp on
@@ -30,6 +31,7 @@ namespace thv {
 namespace {
 
 constexpr int CBS = 256;
+constexpr uint32_t SEG = 64;  // tiles per segment of the two-level scan
 constexpr uint32_t SUB = TILE / (CBS / 64);  // vertices per wave
 constexpr uint32_t ROUNDS = SUB / 64;
 
@@ -39,8 +41,11 @@ struct CompactArgs {
   uint64_t n;
   uint32_t ntiles;
   uint32_t P;
-  uint32_t *tcnt, *tbc;
-  unsigned long long *tdeg, *tbe;
+  uint32_t *tcnt;               // per tile and queue: entries
+  unsigned long long *tdeg;     //   and edges
+  unsigned long long *ssum;     // per segment of SEG tiles: the 4 sums (atomics)
+  unsigned long long *sbase;    //   and their exclusive bases
+  uint32_t nseg;
   uint32_t *qv[2];
   unsigned long long *qe[2];
   uint32_t *qc[2];
@@ -151,6 +156,7 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
       for (int w = 0; w < CBS / 64; w++) x += s[w][threadIdx.x];
       if (threadIdx.x < 2) a.tcnt[2 * tile + threadIdx.x] = (uint32_t)x;
       else a.tdeg[2 * tile + threadIdx.x - 2] = x;
+      if (x) atomicAdd(&a.ssum[(tile / SEG) * 4 + threadIdx.x], x);
     }
     __syncthreads();
   }
@@ -159,98 +165,59 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
 constexpr int SBS = 1024;
 constexpr int SNW = SBS / 64;
 
-// Exclusive tile bases.  Wave w owns the contiguous tile segment
-// [w * per, (w + 1) * per): a first sweep sums it (64 tiles per coalesced
-// load, the first KC loads kept in registers), the waves' sums are scanned
-// in LDS, and a second sweep writes the bases with a wave scan per 64
-// tiles.  KC = 12 keeps 768 tiles per wave (2^25 vertices in all) in
-// registers without spilling.
-constexpr uint32_t KC = 12;
-
-__global__ void __launch_bounds__(SBS) k_tscan(CompactArgs a) {
-  __shared__ unsigned long long sh[SNW][4];
-  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint32_t per = (a.ntiles + SNW - 1) / SNW;
-  const uint32_t s0 = min(a.ntiles, wv * per), s1 = min(a.ntiles, s0 + per);
-  const uint32_t *__restrict__ tcnt = a.tcnt;
-  const unsigned long long *__restrict__ tdeg = a.tdeg;
-  uint32_t cc0[KC], cc1[KC];
-  unsigned long long ce0[KC], ce1[KC];
-  unsigned long long x[4] = {0, 0, 0, 0};
+// Segment bases: the count kernels add each tile's four sums (entries and
+// edges per queue) into its segment of SEG tiles; one block scans the
+// segments (nseg <= 16384: 2^32 vertices), writes the totals, resets the
+// counters and clears the segment sums for the next count.  A tile's base
+// is its segment's base plus the tiles before it in the segment, which the
+// write pass adds itself (one coalesced load).
+__global__ void __launch_bounds__(SBS) k_sscan(CompactArgs a) {
+  __shared__ unsigned long long sh[4][SNW + 1];
+  unsigned long long carry[4] = {0, 0, 0, 0};
+  for (uint32_t b = 0; b < a.nseg; b += SBS) {
+    const uint32_t sg = b + threadIdx.x;
+    unsigned long long x[4] = {0, 0, 0, 0};
+    if (sg < a.nseg) {
 #pragma unroll
-  for (uint32_t k = 0; k < KC; k++) {
-    const uint32_t t = s0 + k * 64 + lane;
-    const bool ok = t < s1;
-    cc0[k] = ok ? tcnt[2 * t] : 0u;
-    cc1[k] = ok ? tcnt[2 * t + 1] : 0u;
-    ce0[k] = ok ? tdeg[2 * t] : 0ull;
-    ce1[k] = ok ? tdeg[2 * t + 1] : 0ull;
-  }
-#pragma unroll
-  for (uint32_t k = 0; k < KC; k++) {
-    x[0] += cc0[k];
-    x[1] += cc1[k];
-    x[2] += ce0[k];
-    x[3] += ce1[k];
-  }
-  for (uint32_t t = s0 + KC * 64 + lane; t < s1; t += 64) {
-    x[0] += tcnt[2 * t];
-    x[1] += tcnt[2 * t + 1];
-    x[2] += tdeg[2 * t];
-    x[3] += tdeg[2 * t + 1];
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) x[i] = wave_sum(x[i]);
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) sh[wv][i] = x[i];
-  }
-  __syncthreads();
-  unsigned long long run[4] = {0, 0, 0, 0};
-  for (uint32_t w = 0; w < wv; w++) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) run[i] += sh[w][i];
-  }
-  // counts are below 2^32 (n is), edge sums need 64 bits
-  auto step = [&](uint32_t t, const unsigned long long (&v)[4]) {
-    const bool ok = t < s1;
-    const uint32_t i0 = wave_incl_scan((uint32_t)v[0]), i1 = wave_incl_scan((uint32_t)v[1]);
-    const unsigned long long i2 = wave_incl_scan(v[2]), i3 = wave_incl_scan(v[3]);
-    if (ok) {
-      a.tbc[2 * t] = (uint32_t)run[0] + i0 - (uint32_t)v[0];
-      a.tbc[2 * t + 1] = (uint32_t)run[1] + i1 - (uint32_t)v[1];
-      a.tbe[2 * t] = run[2] + i2 - v[2];
-      a.tbe[2 * t + 1] = run[3] + i3 - v[3];
+      for (int i = 0; i < 4; i++) {
+        x[i] = a.ssum[sg * 4 + i];
+        a.ssum[sg * 4 + i] = 0;
+      }
     }
-    run[0] += (uint32_t)__shfl((int)i0, 63, 64);
-    run[1] += (uint32_t)__shfl((int)i1, 63, 64);
-    run[2] += __shfl(i2, 63, 64);
-    run[3] += __shfl(i3, 63, 64);
-  };
 #pragma unroll
-  for (uint32_t k = 0; k < KC; k++) {
-    if (s0 + k * 64 >= s1) break;
-    const unsigned long long v[4] = {cc0[k], cc1[k], ce0[k], ce1[k]};
-    step(s0 + k * 64 + lane, v);
+    for (int i = 0; i < 4; i++) {
+      unsigned long long tot;
+      const unsigned long long ex = block_excl_scan<SBS>(x[i], sh[i], &tot);
+      if (sg < a.nseg) a.sbase[sg * 4 + i] = carry[i] + ex;
+      carry[i] += tot;
+    }
+    __syncthreads();
   }
-  for (uint32_t k = KC; s0 + k * 64 < s1; k++) {
-    const uint32_t t = s0 + k * 64 + lane;
-    const bool ok = t < s1;
-    const unsigned long long v[4] = {ok ? tcnt[2 * t] : 0u, ok ? tcnt[2 * t + 1] : 0u,
-                                     ok ? tdeg[2 * t] : 0ull, ok ? tdeg[2 * t + 1] : 0ull};
-    step(t, v);
-  }
-  // the last wave's running sums end at the grand totals
-  if (wv == SNW - 1 && lane == 0) {
-    a.ctr->qn[0] = run[0];
-    a.ctr->qn[1] = run[1];
-    a.ctr->qw[0] = run[2];
-    a.ctr->qw[1] = run[3];
+  if (threadIdx.x == 0) {
+    a.ctr->qn[0] = carry[0];
+    a.ctr->qn[1] = carry[1];
+    a.ctr->qw[0] = carry[2];
+    a.ctr->qw[1] = carry[3];
     a.ctr->touched[0] = 0;
     a.ctr->touched[1] = 0;
     a.ctr->rd[0] = 0;
     a.ctr->rd[1] = 0;
   }
+}
+
+// Base of a tile: its segment's base plus the tiles before it in the
+// segment (one load per lane, then a wave sum).  Every lane gets it.
+__device__ __forceinline__ void tile_bases(const CompactArgs &a, uint32_t tile, uint32_t lane,
+                                           unsigned long long tb[4]) {
+  const uint32_t sg = tile / SEG, t0 = sg * SEG, t = t0 + lane;
+  const bool before = t < tile;
+  unsigned long long x[4];
+  x[0] = before ? a.tcnt[2 * t] : 0u;
+  x[1] = before ? a.tcnt[2 * t + 1] : 0u;
+  x[2] = before ? a.tdeg[2 * t] : 0ull;
+  x[3] = before ? a.tdeg[2 * t + 1] : 0ull;
+#pragma unroll
+  for (int i = 0; i < 4; i++) tb[i] = a.sbase[sg * 4 + i] + wave_sum(x[i]);
 }
 
 // The write pass gives each lane a run of VPL consecutive vertices (the
@@ -268,9 +235,8 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
     if (a.tcnt[2 * tile] == 0 && a.tcnt[2 * tile + 1] == 0) continue;
     const uint64_t v0 = (uint64_t)tile * TILE + (uint64_t)wv * SUB + (uint64_t)lane * VPL;
     const uint32_t sh = (uint32_t)(v0 & 31);
-    // the tile's bases load together with the words
-    const unsigned long long tb[4] = {a.tbc[2 * tile], a.tbc[2 * tile + 1], a.tbe[2 * tile],
-                                      a.tbe[2 * tile + 1]};
+    unsigned long long tb[4];
+    tile_bases(a, tile, lane, tb);
     uint32_t sel = 0, dev = 0;
     if (v0 < a.n) {
       const uint32_t fw = (a.front[v0 >> 5] >> sh) & 0xFFFFu;
@@ -351,6 +317,7 @@ struct ProbeArgs {
   uint32_t ntiles, D, rec;
   uint32_t *tcnt[2];
   unsigned long long *tdeg[2];
+  unsigned long long *ssum[2];
   unsigned long long *cnt;  // [0] hint hits, [1] unvisited, [2] with a hint, [3] words written
   uint32_t *slist;          // short lists: wave (tile, w) writes from slot tile*TILE + w*SUB
   uint32_t *scnt;           //   and their count at [tile * 4 + w]
@@ -379,6 +346,22 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
     const uint32_t vw = owner ? a.vis[w0 + lane] : 0xFFFFFFFFu;
     const uint32_t rw = owner ? a.rbit[w0 + lane] : 0u;
+    // fast path: no vertex of the block's tile is unvisited
+    uint32_t uw = 0;
+    if (owner) {
+      const uint64_t first = (w0 + lane) << 5;
+      uw = ~vw;
+      if (a.n - first < 32) uw &= (1u << (a.n - first)) - 1u;
+    }
+    if (!__syncthreads_or(uw != 0)) {
+      if (lane == 0) a.scnt[tile * (CBS / 64) + wv] = 0;
+      if (threadIdx.x < 8) {
+        const uint32_t set = threadIdx.x >> 2, i = threadIdx.x & 3;
+        if (i < 2) a.tcnt[set][2 * tile + i] = 0;
+        else a.tdeg[set][2 * tile + i - 2] = 0;
+      }
+      continue;
+    }
     uint32_t un = 0, dev = 0;  // bit r: round r's vertex is unvisited / HBM-resident
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
@@ -452,6 +435,7 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
       const uint32_t set = threadIdx.x >> 2, i = threadIdx.x & 3;
       if (i < 2) a.tcnt[set][2 * tile + i] = (uint32_t)y;
       else a.tdeg[set][2 * tile + i - 2] = y;
+      if (y) atomicAdd(&a.ssum[set][(tile / SEG) * 4 + i], y);
     }
     __syncthreads();
   }
@@ -555,8 +539,9 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   a.ntiles = (uint32_t)std::max<uint64_t>(1, div_up(g.n, TILE));
   a.tcnt = w.tcnt[tset].as<uint32_t>();
   a.tdeg = w.tdeg[tset].as<unsigned long long>();
-  a.tbc = w.tbc[tset].as<uint32_t>();
-  a.tbe = w.tbe[tset].as<unsigned long long>();
+  a.ssum = w.ssum[tset].as<unsigned long long>();
+  a.sbase = w.sbase[tset].as<unsigned long long>();
+  a.nseg = (uint32_t)div_up(a.ntiles, SEG);
   for (int q = 0; q < 2; q++) {
     a.qv[q] = w.qv[q].as<uint32_t>();
     a.qe[q] = w.qe[q].as<unsigned long long>();
@@ -581,7 +566,7 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   // the tile bases and totals are still in place
   if (!(mode & CMP_WRITE_ONLY)) {
     hipLaunchKernelGGL(k_tcount, dim3(grid), dim3(CBS), 0, s, a);
-    hipLaunchKernelGGL(k_tscan, dim3(1), dim3(SBS), 0, s, a);
+    hipLaunchKernelGGL(k_sscan, dim3(1), dim3(SBS), 0, s, a);
   }
   if (!(mode & CMP_COUNT_ONLY)) {
     hipLaunchKernelGGL(k_twrite, dim3(grid), dim3(CBS), 0, s, a);
@@ -609,6 +594,7 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   for (int k = 0; k < 2; k++) {
     p.tcnt[k] = w.tcnt[k].as<uint32_t>();
     p.tdeg[k] = w.tdeg[k].as<unsigned long long>();
+    p.ssum[k] = w.ssum[k].as<unsigned long long>();
   }
   p.cnt = cnt;
   p.eptr = g.store.d_eptr.as<uint64_t>();
@@ -640,10 +626,11 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
     a.ntiles = p.ntiles;
     a.tcnt = w.tcnt[k].as<uint32_t>();
     a.tdeg = w.tdeg[k].as<unsigned long long>();
-    a.tbc = w.tbc[k].as<uint32_t>();
-    a.tbe = w.tbe[k].as<unsigned long long>();
+    a.ssum = w.ssum[k].as<unsigned long long>();
+    a.sbase = w.sbase[k].as<unsigned long long>();
+    a.nseg = (uint32_t)div_up(a.ntiles, SEG);
     a.ctr = k == 0 ? cs : cl;
-    hipLaunchKernelGGL(k_tscan, dim3(1), dim3(SBS), 0, s, a);
+    hipLaunchKernelGGL(k_sscan, dim3(1), dim3(SBS), 0, s, a);
   }
   HIPC(hipGetLastError());
 }

@@ -367,8 +367,10 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
     if (tcnt[k].bytes < tiles * 2 * 4) {
       tcnt[k].alloc(tiles * 2 * 4);
       tdeg[k].alloc(tiles * 2 * 8);
-      tbc[k].alloc(tiles * 2 * 4);
-      tbe[k].alloc(tiles * 2 * 8);
+      const uint64_t segs = tiles / 64 + 1;
+      ssum[k].alloc(segs * 4 * 8);
+      HIPC(hipMemset(ssum[k].p, 0, segs * 4 * 8));
+      sbase[k].alloc(segs * 4 * 8);
     }
   }
   if (n == g.n && m == g.m && C == chunk && ctr.p) return;

@@ -128,7 +128,8 @@ struct QueueSet {
   // per tile and queue, in two sets (a bottom-up level counts its short
   // and long unvisited lists in one sweep):
   DevMem tcnt[2], tdeg[2];    // active vertices / edges
-  DevMem tbc[2], tbe[2];      // exclusive bases
+  DevMem ssum[2], sbase[2];   // per segment of 64 tiles: sums (kept zero
+                              // between counts) and exclusive bases
   DevMem ptouch;              // per partition: stamp of the last touching call
   DevMem slist, scnt;         // bottom-up probe: short lists per wave region
   void ensure(const Graph &g, uint32_t C);
