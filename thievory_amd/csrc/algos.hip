@@ -358,10 +358,11 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     } else {
       if (!probed) probe(hint, next);
       HIPC(hipEventRecord(w.ev[0], s));
-      // the probe scanned the short lists already; the long set is the one
-      // it counted
+      // the probe scanned the short lists already (and, with hints, settled
+      // every degree-1 vertex by its hint); the long set is the one it
+      // counted, so the same degree bounds select it here
       launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_NO_CHUNKS | CMP_WRITE_ONLY, nullptr,
-                     0, D + 1, 0xFFFFFFFFu, 1);
+                     0, std::max<uint32_t>(D + 1, hint ? 2u : 1u), 0xFFFFFFFFu, 1);
       BottomUpArgs b;
       b.qv[0] = w.qs.qv[0].as<uint32_t>();
       b.qv[1] = w.qs.qv[1].as<uint32_t>();

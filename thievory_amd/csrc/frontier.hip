@@ -400,17 +400,19 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     uint32_t d[ROUNDS];
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
-    // counts, and the wave's short lists (in vertex order)
+    // counts, and the wave's short lists (in vertex order).  With hints, a
+    // degree-1 vertex is settled already: its hint is its one neighbour.
+    const uint32_t dlo = a.hint ? 2u : 1u;
     unsigned long long x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t ns = 0;
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
       const uint32_t q = (dev >> r) & 1u;
-      const bool sh = d[r] != 0 && d[r] <= a.D;
-      const unsigned long long b = __ballot(sh);
+      const bool sh = d[r] >= dlo && d[r] <= a.D;
+      const unsigned long long b = __ballot(sh);  // every lane takes part
       if (sh) a.slist[base + ns + __popcll(b & lt)] = (uint32_t)(base + r * 64 + lane);
       ns += __popcll(b);
-      if (d[r] == 0) continue;
+      if (d[r] < dlo) continue;
       const uint32_t k = sh ? 0 : 4;
       x[k + q] += 1;
       x[k + 2 + q] += d[r];
