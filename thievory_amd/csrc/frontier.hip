@@ -336,7 +336,7 @@ constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
 //   Counts: tile set 0 holds the short lists (entries, records), tile set
 //   1 the long unvisited lists (degree > D) left for the early-exit kernel,
 //   both split by residency.
-__global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
+__global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   __shared__ unsigned long long s[CBS / 64][8];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = lanemask_lt();
@@ -403,7 +403,8 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
     // counts, and the wave's short lists (in vertex order).  With hints, a
     // degree-1 vertex is settled already: its hint is its one neighbour.
     const uint32_t dlo = a.hint ? 2u : 1u;
-    unsigned long long x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
+    unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
     uint32_t ns = 0;
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
@@ -413,9 +414,9 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
       if (sh) a.slist[base + ns + __popcll(b & lt)] = (uint32_t)(base + r * 64 + lane);
       ns += __popcll(b);
       if (d[r] < dlo) continue;
-      const uint32_t k = sh ? 0 : 4;
-      x[k + q] += 1;
-      x[k + 2 + q] += d[r];
+      const uint32_t k = (sh ? 0 : 2) + q;
+      xc[k] += 1;
+      xe[k] += d[r];
     }
     if (lane == 0) a.scnt[tile * (CBS / 64) + wv] = ns;
     hits += __popc(m);  // hint hits
@@ -424,11 +425,15 @@ __global__ void __launch_bounds__(CBS) k_bu_probe(ProbeArgs a) {
       a.next[w0 + lane] |= m;
       nwords++;
     }
+    // s[w][set * 4 + i]: i = 0, 1 entries (host, HBM), 2, 3 edges
 #pragma unroll
-    for (int i = 0; i < 8; i++) x[i] = wave_sum(x[i]);
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < 8; i++) s[wv][i] = x[i];
+    for (int k = 0; k < 4; k++) {
+      const uint32_t c = wave_sum(xc[k]);
+      const unsigned long long e = wave_sum(xe[k]);
+      if (lane == 0) {
+        s[wv][(k >> 1) * 4 + (k & 1)] = c;
+        s[wv][(k >> 1) * 4 + 2 + (k & 1)] = e;
+      }
     }
     __syncthreads();
     if (threadIdx.x < 8) {
