@@ -228,7 +228,7 @@ __device__ __forceinline__ void tile_bases(const CompactArgs &a, uint32_t tile, 
 constexpr uint32_t VPL = SUB / 64;
 static_assert(VPL == 16, "a lane's run is half a bitmap word");
 
-__global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
+__global__ void __launch_bounds__(CBS, 4) k_twrite(CompactArgs a) {
   __shared__ unsigned long long s[CBS / 64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
       sel = (a.invert ? ~fw : fw) & 0xFFFFu;
       if (a.n - v0 < VPL) sel &= (1u << (a.n - v0)) - 1u;
     }
-    uint32_t d[VPL], part[VPL];
+    uint32_t d[VPL];
 #pragma unroll
     for (uint32_t r = 0; r < VPL; r++) d[r] = ((sel >> r) & 1u) ? a.deg[v0 + r] : 0u;
 #pragma unroll
@@ -255,9 +255,13 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
       }
     }
     dev &= sel;
-#pragma unroll
-    for (uint32_t r = 0; r < VPL; r++)
-      part[r] = ((sel >> r) & 1u) ? a.vpart[v0 + r] : 0xFFFFFFFFu;
+    // partitions of the lane's first and last selected vertices (a run of
+    // 16 vertices rarely spans two partitions; then each is looked up)
+    uint32_t pf = 0xFFFFFFFFu, pl = 0xFFFFFFFFu;
+    if (sel) {
+      pf = a.vpart[v0 + (__ffs(sel) - 1)];
+      pl = a.vpart[v0 + (31 - __clz(sel))];
+    }
     // lane totals -> exclusive offsets within the wave, then the block
     const unsigned long long cd = __popc(dev), ch = __popc(sel) - cd;
     unsigned long long ed = 0, eh = 0;
@@ -283,6 +287,9 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
       eb[0] += s[w][2];
       eb[1] += s[w][3];
     }
+    // partitions touched: a plain store of this call's stamp (every writer
+    // writes the same value); k_touched counts the stamps afterwards
+    if (sel && pf == pl) a.ptouch[pf] = a.stamp;
     uint32_t prevp = 0xFFFFFFFFu;
 #pragma unroll
     for (uint32_t r = 0; r < VPL; r++) {
@@ -297,12 +304,12 @@ __global__ void __launch_bounds__(CBS) k_twrite(CompactArgs a) {
         for (unsigned long long c = (e0 + a.C - 1) / a.C; c * a.C < e0 + d[r]; c++)
           qc[c] = (uint32_t)i;
       }
-      // partitions touched: a plain store of this call's stamp wherever
-      // the lane's partition changes (every writer writes the same value);
-      // k_touched counts the stamps afterwards
-      if (part[r] != prevp) {
-        a.ptouch[part[r]] = a.stamp;
-        prevp = part[r];
+      if (pf != pl) {
+        const uint32_t p = a.vpart[v0 + r];
+        if (p != prevp) {
+          a.ptouch[p] = a.stamp;
+          prevp = p;
+        }
       }
     }
     __syncthreads();
