@@ -548,6 +548,12 @@ void run_pr(Graph &g, int mode, float alpha, float tol, int max_iter, float *out
   ExpandArgs a = base_args(g);
   a.pr_src = f1;  // push: delta, pull: contribution
   a.pr_dst = f2;  // push: residual, pull: sum
+  if (mode == THV_PR_PUSH) {
+    ensure_pr_hubs(g);
+    a.hubs = g.d_hubs.as<uint32_t>();
+    a.hubbit = g.d_hubbit.as<uint32_t>();
+    a.nhubs = g.nhubs;
+  }
   LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
   unsigned long long *aux = w.aux.as<unsigned long long>();
   const Algo algo = mode == THV_PR_PUSH ? A_PR_PUSH : A_PR_PULL;

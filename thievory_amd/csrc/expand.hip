@@ -27,7 +27,8 @@
 //          (bfs_kernels.cu:29-44)
 //   CC   : atomicMin(val[w], val[v]) (cc_kernels.cu:50-66)
 //   SSSP : atomicMin(val[w], val[v] + weight) (sssp_kernels.cu:124-141)
-//   PR   : push atomicAdd(res[w], delta[v]) (pr_kernels.cu:539-542);
+//   PR   : push atomicAdd(res[w], delta[v]) (pr_kernels.cu:539-542), into a
+//          per-block LDS sum first when w is one of the hub destinations;
 //          pull sum[v] += contrib[u] over in-edges (pr_kernels.cu:45-51)
 //   HINT  (store set-up, not a traversal): key64[v] = max over the edges
 //          of (deg[w] << 32 | ~w), reduced per entry in LDS
@@ -36,6 +37,12 @@
 namespace thv {
 
 constexpr int EBS = 256;
+
+// Slot of a hub id in the PR-push LDS table (Fibonacci hashing).
+__device__ __forceinline__ uint32_t hub_slot(uint32_t w) {
+  static_assert((PR_HUB_SLOTS & (PR_HUB_SLOTS - 1)) == 0, "power-of-two table");
+  return (w * 2654435761u) >> (32 - __builtin_ctz(PR_HUB_SLOTS));
+}
 
 template <int ALGO, int REC, int EPT>
 __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
@@ -46,9 +53,27 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   __shared__ unsigned long long s_base[C + 1];
   __shared__ uint32_t s_u[C + 1];  // CC/SSSP: source value; PR pull: vertex id
   __shared__ float s_f[C + 1];     // PR push: delta; PR pull: partial sum
+  // PR push: per-block sums for the hub destinations (an open-addressing
+  // table of the hub ids, built at kernel start, flushed at the end)
+  constexpr uint32_t HS = ALGO == A_PR_PUSH ? PR_HUB_SLOTS : 1;
+  __shared__ uint32_t s_hk[HS];
+  __shared__ float s_hv[HS];
 
   const ChunkSrc cs = chunk_src(a.ctr, C);
   const uint32_t tid = threadIdx.x;
+  if (ALGO == A_PR_PUSH) {
+    for (uint32_t t = tid; t < HS; t += EBS) {
+      s_hk[t] = 0xFFFFFFFFu;
+      s_hv[t] = 0.0f;
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < a.nhubs; t += EBS) {
+      const uint32_t w = a.hubs[t];
+      uint32_t sl = hub_slot(w);
+      while (atomicCAS(&s_hk[sl], 0xFFFFFFFFu, w) != 0xFFFFFFFFu) sl = (sl + 1) & (HS - 1);
+    }
+    __syncthreads();
+  }
 
   for (uint32_t c = blockIdx.x; c < cs.total; c += gridDim.x) {
     int q;
@@ -155,9 +180,24 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
         if (!(nw[j] & bit)) atomicOr(&a.next[w >> 5], bit);
       }
     } else if (ALGO == A_PR_PUSH) {
+      // a hub destination adds into the block's LDS table (it is there:
+      // every hub was inserted), any other one atomically in HBM
+      uint32_t hb[EPT];
 #pragma unroll
-      for (int j = 0; j < EPT; j++)
-        if (dst[j] != 0xFFFFFFFFu) unsafeAtomicAdd(&a.pr_dst[dst[j]], s_f[ent[j]]);
+      for (int j = 0; j < EPT; j++) hb[j] = a.hubbit[dst[j] != 0xFFFFFFFFu ? dst[j] >> 5 : 0];
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const uint32_t w = dst[j];
+        if (w == 0xFFFFFFFFu) continue;
+        const float d = s_f[ent[j]];
+        if ((hb[j] >> (w & 31)) & 1u) {
+          uint32_t sl = hub_slot(w);
+          while (s_hk[sl] != w) sl = (sl + 1) & (HS - 1);
+          atomicAdd(&s_hv[sl], d);
+        } else {
+          unsafeAtomicAdd(&a.pr_dst[w], d);
+        }
+      }
     } else if (ALGO == A_PR_PULL) {  // dst[j] is an in-neighbour of the entry's vertex
       float c[EPT];
 #pragma unroll
@@ -188,6 +228,11 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       }
     }
     __syncthreads();
+  }
+  if (ALGO == A_PR_PUSH) {
+    __syncthreads();
+    for (uint32_t t = tid; t < HS; t += EBS)
+      if (s_hk[t] != 0xFFFFFFFFu && s_hv[t] != 0.0f) unsafeAtomicAdd(&a.pr_dst[s_hk[t]], s_hv[t]);
   }
 }
 

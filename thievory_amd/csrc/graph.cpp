@@ -351,6 +351,38 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   if (s.hint_reserved && s.host_vertices > 0) build_parent_hints(g);
 }
 
+// PageRank push hubs: count the destinations of every 64th record (host
+// copy of the edge array), keep the PR_HUBS most frequent ones that were
+// seen at least 4 times (256 records expected), sorted.
+void ensure_pr_hubs(Graph &g) {
+  if (g.hubs_built) return;
+  std::vector<uint32_t> cnt(g.n, 0);
+  const char *rec = static_cast<const char *>(g.h_edges.p);
+  for (uint64_t e = 0; e < g.m; e += 64) {
+    uint32_t w;
+    memcpy(&w, rec + e * g.rec, 4);
+    if (w < g.n) cnt[w]++;
+  }
+  std::vector<uint32_t> cand;
+  for (uint64_t v = 0; v < g.n; v++)
+    if (cnt[v] >= 4) cand.push_back((uint32_t)v);
+  if (cand.size() > PR_HUBS) {
+    std::nth_element(cand.begin(), cand.begin() + PR_HUBS, cand.end(),
+                     [&](uint32_t a, uint32_t b) { return cnt[a] > cnt[b]; });
+    cand.resize(PR_HUBS);
+  }
+  std::sort(cand.begin(), cand.end());
+  std::vector<uint32_t> bits(bitmap_words(g.n), 0u);
+  for (uint32_t v : cand) bits[v >> 5] |= 1u << (v & 31);
+  g.d_hubs.alloc(std::max<size_t>(1, cand.size()) * 4);
+  if (!cand.empty())
+    HIPC(hipMemcpy(g.d_hubs.p, cand.data(), cand.size() * 4, hipMemcpyHostToDevice));
+  g.d_hubbit.alloc(bits.size() * 4);
+  HIPC(hipMemcpy(g.d_hubbit.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice));
+  g.nhubs = (uint32_t)cand.size();
+  g.hubs_built = true;
+}
+
 void QueueSet::ensure(const Graph &g, uint32_t chunk) {
   if (P != g.store.P || !ptouch.p) {
     ptouch.alloc((uint64_t)(g.store.P + 1) * 4);

@@ -187,6 +187,12 @@ struct Graph {
   // in full by the edge-balanced kernel, longer lists by the early-exit one
   uint32_t do_pull_degree = 16;
   bool parent_hints = true;
+  // PageRank push: the destinations that receive the most records (from a
+  // 1-in-64 sample of the edge array), sorted, and their bitmap; blocks
+  // accumulate adds to them in LDS.  Built on the first push call.
+  DevMem d_hubs, d_hubbit;
+  uint32_t nhubs = 0;
+  bool hubs_built = false;
   ~Graph();
 };
 
@@ -220,6 +226,9 @@ struct ExpandArgs {
   unsigned long long *key64;  // hint build: per-vertex best key
   float *pr_src;            // PR push: delta; PR pull: contribution
   float *pr_dst;            // PR push: residual; PR pull: sum
+  const uint32_t *hubs;     // PR push: hub ids (nhubs, sorted) and bitmap
+  const uint32_t *hubbit;
+  uint32_t nhubs;
   uint32_t level_value;     // BFS: value given to discovered vertices
   uint32_t C;
   int rec;
@@ -271,6 +280,9 @@ inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);
 
 // algos.hip
+constexpr uint32_t PR_HUBS = 2048;       // hub table entries
+constexpr uint32_t PR_HUB_SLOTS = 4096;  // LDS hash slots (power of two)
+void ensure_pr_hubs(Graph &g);
 void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
 void run_cc(Graph &g, uint32_t *out, thv_stats *st);
 void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
