@@ -153,6 +153,11 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       partitions.  Changing it, SYMMETRIC or DIRECTION
  *                       rebuilds the store with the same budget.
  *   THV_OPT_HINT_BYTES  (read only) HBM bytes the hints use, 0 if none.
+ *   THV_OPT_SSSP_DELTA  SSSP bucket width: vertices wait in distance buckets
+ *                       of this width before their edges are relaxed
+ *                       (delta-stepping; default 16).  0: frontier
+ *                       Bellman-Ford, the reference's order.  Distances are
+ *                       the same either way.
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -163,6 +168,7 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_DO_PULL_DEGREE 7
 #define THV_OPT_PARENT_HINTS 8
 #define THV_OPT_HINT_BYTES 9
+#define THV_OPT_SSSP_DELTA 10
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

@@ -243,6 +243,31 @@ def test_symmetric_hint_on_arrays():
         gc.set_option("symmetric", 1)
 
 
+@pytest.mark.parametrize("frac", [1.0, 0.25])
+def test_sssp_delta_stepping(rmat16, frac):
+    """Delta-stepping (any bucket width) reaches the frontier Bellman-Ford
+    fixpoint: the distances equal the oracle's Dijkstra bit for bit; wider
+    buckets relax more edges, width 0 is the reference's order."""
+    g, og = rmat16
+    _say("case: sssp delta-stepping", frac)
+    g.store_config(int(frac * g.m * g.edge_bytes), 4096)
+    try:
+        for s in (1, 7, 65535):
+            ref = O.sssp(og, s)
+            edges = {}
+            for delta in (0, 1, 7, 32, 1000, 1 << 31):
+                g.set_option("sssp_delta", delta)
+                d, st = g.sssp(s)
+                assert np.array_equal(d, ref), (s, delta)
+                edges[delta] = st.edges_traversed
+            # every bucket width relaxes each reached vertex at least once
+            reached = ref != og.n + 1
+            assert edges[1] >= int(og.degrees()[reached].sum())
+    finally:
+        g.set_option("sssp_delta", 16)
+    assert g.get_option("sssp_delta") == 16.0
+
+
 def test_pr_push_pull_rmat():
     _say("case: rmat14 directed pagerank")
     og = O.rmat(14, 16, seed=2, symmetric=False)

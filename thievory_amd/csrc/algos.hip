@@ -494,6 +494,69 @@ void run_cc(Graph &g, uint32_t *out, thv_stats *st) {
   finish_values(g, out, st, 0xFFFFFFFFu);
 }
 
+// SSSP by delta-stepping (Meyer & Sanders 2003) over the same compaction and
+// expansion kernels: front[0] is the pending set (vertices whose value
+// improved and whose edges were not relaxed since).  Each iteration takes
+// the pending vertices with value below the bucket bound T (the write pass
+// clears them from the set), relaxes their edges, and the expansion puts
+// every improved vertex back.  When no pending vertex is below T, T moves
+// up by delta; the loop ends with the set empty.  The fixpoint is the
+// frontier Bellman-Ford one (shortest distances are unique); fewer edges
+// are relaxed twice because vertices wait for their bucket.
+static void sssp_delta(Graph &g, thv_stats *st) {
+  Workspace &w = g.ws;
+  hipStream_t s = g.stream;
+  const uint64_t words = bitmap_words(g.n);
+  ExpandArgs a = base_args(g);
+  LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
+  uint32_t *pend = w.front[0].as<uint32_t>();
+  const uint32_t *val = w.val.as<uint32_t>();
+  a.next = pend;
+  const uint64_t delta = g.sssp_delta;
+  uint64_t thr = 1 + delta;  // values are 1 + distance
+  double expand_ms = 0;
+  uint32_t launches = 0;
+  const uint64_t t0 = now_ns();
+  for (uint32_t it = 0;; it++) {
+    LevelCtr *c = d_ctr + (it & 1);
+    const uint32_t t32 = (uint32_t)std::min<uint64_t>(thr, 0xFFFFFFFFull);
+    launch_compact(g, w.qs, pend, c, s, 0, nullptr, 0, 1, 0xFFFFFFFFu, 0, val, t32, pend);
+    HIPC(hipMemcpyAsync(&w.h_ctr[it & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const LevelCtr h = w.h_ctr[it & 1];
+    if (h.qn[0] + h.qn[1] == 0) {
+      // bucket done: the smallest pending value opens the next one (none
+      // pending: done)
+      uint32_t *d_min = reinterpret_cast<uint32_t *>(w.aux.as<unsigned long long>() + 3);
+      launch_fill_u32(d_min, 0xFFFFFFFFu, 1, s);
+      launch_compact(g, w.qs, pend, c, s, CMP_COUNT_ONLY, nullptr, 0, 1, 0xFFFFFFFFu, 0, val,
+                     0xFFFFFFFFu, nullptr, d_min);
+      HIPC(hipMemcpyAsync(w.h_aux + 3, d_min, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      const uint32_t lo = reinterpret_cast<const uint32_t *>(w.h_aux + 3)[0];
+      if (lo == 0xFFFFFFFFu) break;
+      thr = (uint64_t)lo + delta;
+      continue;
+    }
+    a.ctr = c;
+    HIPC(hipEventRecord(w.ev[0], s));
+    launch_expand(g, A_SSSP, a, s);
+    HIPC(hipEventRecord(w.ev[1], s));
+    HIPC(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    expand_ms += ms;
+    launches++;
+    stats_level(g, st, h);
+  }
+  if (st) {
+    st->ms = (now_ns() - t0) * 1e-6;
+    st->expand_ms = expand_ms;
+    st->expand_launches = launches;
+  }
+  (void)words;
+}
+
 void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   check_src(g, src);
   if (g.rec != 8) fail(THV_ENOTSUP, "SSSP needs a weighted graph (8-byte records)");
@@ -507,7 +570,8 @@ void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   HIPC(hipMemsetAsync(w.front[0].p, 0, words * 4, s));
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
   HIPC(hipStreamSynchronize(s));
-  value_loop(g, A_SSSP, st);
+  if (g.sssp_delta) sssp_delta(g, st);
+  else value_loop(g, A_SSSP, st);
   finish_values(g, out, st, (uint32_t)(g.n + 1));
 }
 

@@ -58,6 +58,10 @@ struct CompactArgs {
   uint32_t *val;       // BFS: frontier vertices' level, written by k_tcount
   uint32_t val_value;
   uint32_t dmin, dmax; // only vertices with dmin <= degree <= dmax (dmin >= 1)
+  const uint32_t *fval;  // if set: only vertices with fval[v] < fthr
+  uint32_t fthr;
+  uint32_t *clear;       // if set (write pass): clear the selected vertices' bits here
+  uint32_t *minout;      // if set (count pass, with fval): atomicMin of the selected fval
 };
 
 // One wave's 1024-vertex sub-tile, read in two dependent memory round
@@ -66,6 +70,7 @@ struct CompactArgs {
 // lane shuffle, then the degrees of all selected vertices of all rounds are
 // loaded together.  Round r of lane l is vertex base + 64 r + l.
 struct SubTile {
+  uint32_t fmin;       // smallest fval of a selected vertex (with fval), else ~0
   uint32_t act;        // bit r: vertex of round r is selected and its degree in range
   uint32_t dev;        // bit r: its edges are in HBM
   uint32_t raw;        // bit r: its frontier bit is set (before any inversion)
@@ -96,6 +101,17 @@ __device__ __forceinline__ void load_subtile(const CompactArgs &a, uint64_t base
 #pragma unroll
   for (uint32_t r = 0; r < ROUNDS; r++)
     t.d[r] = ((t.act >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
+  uint32_t fv[ROUNDS];
+  if (a.fval) {
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++) {
+      fv[r] = 0xFFFFFFFFu;
+      if ((t.act >> r) & 1u) fv[r] = a.fval[base + r * 64 + lane];
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++)
+      if (fv[r] >= a.fthr) t.d[r] = 0;  // fails the value filter
+  }
 #pragma unroll
   for (uint32_t r = 0; r < ROUNDS; r++) {
     if (t.d[r] < a.dmin || t.d[r] > a.dmax) {
@@ -104,6 +120,12 @@ __device__ __forceinline__ void load_subtile(const CompactArgs &a, uint64_t base
     }
   }
   t.dev &= t.act;
+  t.fmin = 0xFFFFFFFFu;
+  if (a.fval) {
+#pragma unroll
+    for (uint32_t r = 0; r < ROUNDS; r++)
+      if ((t.act >> r) & 1u) t.fmin = min(t.fmin, fv[r]);
+  }
 }
 
 // Per-lane totals of a sub-tile: entries and edges per queue.
@@ -119,6 +141,7 @@ __device__ __forceinline__ void subtile_sums(const SubTile &t, uint32_t c[2],
 __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
   __shared__ unsigned long long s[CBS / 64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t fmin = 0xFFFFFFFFu;
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB;
     unsigned long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
@@ -136,6 +159,7 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
         for (uint32_t r = 0; r < ROUNDS; r++)
           if ((t.raw >> r) & 1u) a.val[base + r * 64 + lane] = a.val_value;
       }
+      fmin = min(fmin, t.fmin);
       uint32_t c[2];
       unsigned long long e[2];
       subtile_sums(t, c, e);
@@ -159,6 +183,11 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
       if (x) atomicAdd(&a.ssum[(tile / SEG) * 4 + threadIdx.x], x);
     }
     __syncthreads();
+  }
+  if (a.minout) {  // one atomic per wave that saw a selected vertex
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) fmin = min(fmin, (uint32_t)__shfl_xor((int)fmin, d, 64));
+    if (lane == 0 && fmin != 0xFFFFFFFFu) atomicMin(a.minout, fmin);
   }
 }
 
@@ -247,6 +276,17 @@ __global__ void __launch_bounds__(CBS, 4) k_twrite(CompactArgs a) {
     uint32_t d[VPL];
 #pragma unroll
     for (uint32_t r = 0; r < VPL; r++) d[r] = ((sel >> r) & 1u) ? a.deg[v0 + r] : 0u;
+    if (a.fval) {
+      uint32_t fv[VPL];
+#pragma unroll
+      for (uint32_t r = 0; r < VPL; r++) {
+        fv[r] = 0;
+        if ((sel >> r) & 1u) fv[r] = a.fval[v0 + r];
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < VPL; r++)
+        if (fv[r] >= a.fthr) d[r] = 0;  // fails the value filter
+    }
 #pragma unroll
     for (uint32_t r = 0; r < VPL; r++) {
       if (d[r] < a.dmin || d[r] > a.dmax) {
@@ -255,6 +295,9 @@ __global__ void __launch_bounds__(CBS, 4) k_twrite(CompactArgs a) {
       }
     }
     dev &= sel;
+    // the selected vertices leave the bitmap (two lanes share a word: each
+    // clears its own half)
+    if (a.clear && sel) atomicAnd(&a.clear[v0 >> 5], ~(sel << sh));
     // partitions of the lane's first and last selected vertices (a run of
     // 16 vertices rarely spans two partitions; then each is looked up)
     uint32_t pf = 0xFFFFFFFFu, pl = 0xFFFFFFFFu;
@@ -538,7 +581,8 @@ __global__ void __launch_bounds__(CBS) k_touched(CompactArgs a) {
 
 void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode, uint32_t *val, uint32_t val_value, uint32_t dmin,
-                    uint32_t dmax, int tset) {
+                    uint32_t dmax, int tset, const uint32_t *fval, uint32_t fthr,
+                    uint32_t *clear, uint32_t *minout) {
   if (dmin == 0) dmin = 1;  // degree-0 vertices never have work
   Store &st = g.store;
   CompactArgs a;
@@ -575,6 +619,10 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   a.val_value = val_value;
   a.dmin = dmin;
   a.dmax = dmax;
+  a.fval = fval;
+  a.fthr = fthr;
+  a.clear = clear;
+  a.minout = minout;
   const uint32_t grid = a.ntiles < 65535u ? a.ntiles : 65535u;
   // WRITE_ONLY continues a COUNT_ONLY call on the same bitmap and counters:
   // the tile bases and totals are still in place

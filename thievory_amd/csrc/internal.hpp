@@ -187,6 +187,8 @@ struct Graph {
   // in full by the edge-balanced kernel, longer lists by the early-exit one
   uint32_t do_pull_degree = 16;
   bool parent_hints = true;
+  // SSSP: bucket width of delta-stepping (0: frontier Bellman-Ford)
+  uint32_t sssp_delta = 16;
   // PageRank push: the destinations that receive the most records (from a
   // 1-in-64 sample of the edge array), sorted, and their bitmap; blocks
   // accumulate adds to them in LDS.  Built on the first push call.
@@ -241,10 +243,16 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 // val (BFS only): every vertex with its bit set gets val_value, written in
 // the counting pass.
 // tset: which tile-table set holds the counts (WRITE_ONLY reads it).
+// fval / fthr: select only vertices with fval[v] < fthr.  clear: the write
+// pass clears the bits of the vertices it selects there (the front bitmap
+// itself, for a pending set).  minout (with fval): the count pass lowers
+// *minout to the smallest fval of a selected vertex.
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode = 0, uint32_t *val = nullptr,
                     uint32_t val_value = 0, uint32_t dmin = 1,
-                    uint32_t dmax = 0xFFFFFFFFu, int tset = 0);
+                    uint32_t dmax = 0xFFFFFFFFu, int tset = 0,
+                    const uint32_t *fval = nullptr, uint32_t fthr = 0,
+                    uint32_t *clear = nullptr, uint32_t *minout = nullptr);
 // Bottom-up probe, one sweep over the vertices: every unvisited vertex
 // whose parent hint (hint may be null) is in front joins vis and next, and
 // so does every unvisited vertex of degree 1..D with a frontier vertex in
