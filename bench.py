@@ -271,6 +271,13 @@ def sssp_bench(T, args):
            "vertices": g.n, "edges": g.m, "store": g.store_info()}
     out.update(step_summary(stats, time.perf_counter() - t0, "k_expand<SSSP>"))
     dist, _ = g.sssp(src)
+    # algorithm-independent rate: edges out of the reached vertices per
+    # second (delta-stepping relaxes fewer edges than Bellman-Ford, so the
+    # processed-edge rate alone would understate a faster run)
+    reached_edges = int(deg[dist != g.n + 1].sum())
+    out["reached_edges"] = reached_edges
+    out["reached_edge_gteps"] = round(reached_edges / out["ms_median"] / 1e6, 4)
+    out["sssp_delta"] = g.get_option("sssp_delta")
     if not args.no_cpu:
         from oracle import oracle as O
         off, dst, w, _ = g.export()
