@@ -407,9 +407,13 @@ def main():
     }
     if not args.no_cc:
         cc, cst = g.cc()
+        # union-find CC reads a small part of the edges: the graph's edge
+        # count over the time is the algorithm-independent rate
         out["cc"] = {"ms": round(cst.ms, 2), "edges_processed": cst.edges_traversed,
                      "iterations": cst.iterations,
                      "gteps": round(cst.edges_traversed / cst.ms / 1e6, 3),
+                     "graph_edges_per_s_g": round(g.m / cst.ms / 1e6, 3),
+                     "method": "union-find" if g.get_option("cc_union") else "label propagation",
                      "host_bytes": cst.host_bytes,
                      "mixed_roofline_frac": round(
                          mixed_roofline_ms(cst.hbm_bytes, cst.host_bytes) / cst.ms, 4)}

@@ -158,6 +158,10 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       (delta-stepping; default 16).  0: frontier
  *                       Bellman-Ford, the reference's order.  Distances are
  *                       the same either way.
+ *   THV_OPT_CC_UNION    1 (default): CC of a symmetric graph by union-find
+ *                       with neighbour sampling (the labels are the
+ *                       component minima, as label propagation gives);
+ *                       0: label propagation, as in the reference.
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -169,6 +173,7 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_PARENT_HINTS 8
 #define THV_OPT_HINT_BYTES 9
 #define THV_OPT_SSSP_DELTA 10
+#define THV_OPT_CC_UNION 11
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

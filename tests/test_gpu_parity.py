@@ -268,6 +268,29 @@ def test_sssp_delta_stepping(rmat16, frac):
     assert g.get_option("sssp_delta") == 16.0
 
 
+@pytest.mark.parametrize("scale,ef,frac", [(16, 16, 1.0), (16, 16, 0.25), (14, 1, 0.5),
+                                           (12, 2, 0.0)])
+def test_cc_union_find(scale, ef, frac):
+    """Union-find CC of a symmetric graph gives the component minima, the
+    labels of min-label propagation: equal to the oracle and to the
+    propagation path, on a giant-component R-MAT and on sparse ones with
+    many components."""
+    _say("case: cc union-find", scale, ef, frac)
+    g = T.Graph.rmat(scale, ef, seed=5, symmetric=True)
+    og = O.rmat(scale, ef, seed=5)
+    g.store_config(int(frac * g.m * g.edge_bytes), 1024)
+    ref = O.cc(og)
+    assert g.get_option("cc_union") == 1.0
+    lab, st = g.cc()
+    assert np.array_equal(lab, ref)
+    assert st.edges_traversed <= g.m
+    g.set_option("cc_union", 0)
+    lab2, st2 = g.cc()
+    assert np.array_equal(lab2, ref)
+    g.set_option("cc_union", 1)
+    assert len(np.unique(ref)) > 1
+
+
 def test_pr_push_pull_rmat():
     _say("case: rmat14 directed pagerank")
     og = O.rmat(14, 16, seed=2, symmetric=False)

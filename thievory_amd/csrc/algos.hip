@@ -487,6 +487,17 @@ void run_cc(Graph &g, uint32_t *out, thv_stats *st) {
     hipLaunchKernelGGL(k_iota_u32, dim3(fill_grid(g.n)), dim3(256), 0, s, w.val.as<uint32_t>(),
                        g.n);
   }
+  if (g.sym && g.cc_union && g.n) {
+    HIPC(hipStreamSynchronize(s));
+    if (cc_union_find(g, st)) {
+      finish_values(g, out, st, 0xFFFFFFFFu);
+      return;
+    }
+    // a link hit its step guard: start over with label propagation
+    stats_begin(st);
+    hipLaunchKernelGGL(k_iota_u32, dim3(fill_grid(g.n)), dim3(256), 0, s, w.val.as<uint32_t>(),
+                       g.n);
+  }
   hipLaunchKernelGGL(k_bits_prefix, dim3(fill_grid(words)), dim3(256), 0, s,
                      w.front[0].as<uint32_t>(), g.n, words);
   HIPC(hipStreamSynchronize(s));

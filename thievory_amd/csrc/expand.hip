@@ -32,7 +32,9 @@
 //          pull sum[v] += contrib[u] over in-edges (pr_kernels.cu:45-51)
 //   HINT  (store set-up, not a traversal): key64[v] = max over the edges
 //          of (deg[w] << 32 | ~w), reduced per entry in LDS
+//   CC link (union-find CC of a symmetric graph, cc.hip): link(v, w)
 #include "chunk.hpp"
+#include "uf.hpp"
 
 namespace thv {
 
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
           s_u[t] = v;
           s_f[t] = 0.0f;
         }
-        if (ALGO == A_HINT) s_u[t] = v;
+        if (ALGO == A_HINT || ALGO == A_CC_LINK) s_u[t] = v;
       }
     }
     __syncthreads();
@@ -205,6 +207,10 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
 #pragma unroll
       for (int j = 0; j < EPT; j++)
         if (dst[j] != 0xFFFFFFFFu) atomicAdd(&s_f[ent[j]], c[j]);
+    } else if (ALGO == A_CC_LINK) {
+#pragma unroll
+      for (int j = 0; j < EPT; j++)
+        if (dst[j] != 0xFFFFFFFFu) uf_link(s_u[ent[j]], dst[j], a.val, a.err);
     } else if (ALGO == A_HINT) {
       uint32_t dw[EPT];
 #pragma unroll
@@ -270,6 +276,9 @@ void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
       break;
     case A_HINT:
       r8 ? launch_t<A_HINT, 8>(g, args, s) : launch_t<A_HINT, 4>(g, args, s);
+      break;
+    case A_CC_LINK:
+      r8 ? launch_t<A_CC_LINK, 8>(g, args, s) : launch_t<A_CC_LINK, 4>(g, args, s);
       break;
   }
   HIPC(hipGetLastError());

@@ -189,6 +189,7 @@ struct Graph {
   bool parent_hints = true;
   // SSSP: bucket width of delta-stepping (0: frontier Bellman-Ford)
   uint32_t sssp_delta = 16;
+  bool cc_union = true;       // CC of a symmetric graph by union-find
   // PageRank push: the destinations that receive the most records (from a
   // 1-in-64 sample of the edge array), sorted, and their bitmap; blocks
   // accumulate adds to them in LDS.  Built on the first push call.
@@ -210,9 +211,10 @@ uint64_t now_ns();
 
 // kernels (frontier.hip / expand.hip / algos.hip)
 // A_HINT: parent-hint build -- key64[v] = max over v's edges (v, w) of
-// (deg[w] << 32 | ~w).
+// (deg[w] << 32 | ~w).  A_CC_LINK: union-find link of every edge (val is
+// the parent array, err the step-guard flag).
 enum Algo {
-  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_HINT = 6
+  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_HINT = 6, A_CC_LINK = 7
 };
 
 struct ExpandArgs {
@@ -226,6 +228,7 @@ struct ExpandArgs {
   uint32_t *next;           // next frontier bitmap
   const uint32_t *deg;      // hint build: degrees
   unsigned long long *key64;  // hint build: per-vertex best key
+  uint32_t *err;            // CC link: step-guard flag
   float *pr_src;            // PR push: delta; PR pull: contribution
   float *pr_dst;            // PR push: residual; PR pull: sum
   const uint32_t *hubs;     // PR push: hub ids (nhubs, sorted) and bitmap
@@ -293,6 +296,9 @@ constexpr uint32_t PR_HUB_SLOTS = 4096;  // LDS hash slots (power of two)
 void ensure_pr_hubs(Graph &g);
 void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
 void run_cc(Graph &g, uint32_t *out, thv_stats *st);
+// cc.hip: components of a symmetric graph into ws.val; false when a link
+// hit its step guard (the caller then uses label propagation)
+bool cc_union_find(Graph &g, thv_stats *st);
 void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
 void run_pr(Graph &g, int mode, float damping, float tol, int max_iter,
             float *out, thv_stats *st);
