@@ -89,3 +89,19 @@ def test_cli_bad_file(cli, tmp_path):
                        text=True, timeout=60)
     assert r.returncode == 1 and "EFORMAT" not in r.stdout
     assert "file size" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_symmetric_flag(cli, tmp_path):
+    """--symmetric 1 on an undirected file (every edge both ways): bottom-up
+    BFS levels and union-find CC give the oracle's values."""
+    (tmp_path / "results").mkdir()
+    og = O.rmat(12, 8, seed=21)
+    path = str(tmp_path / "sym.csr")
+    O.write_graph(path, "csr", og)
+    for src in (0, 1, 777):
+        _run(cli, tmp_path, "--input", path, "--algo", "bfs", "--source", src,
+             "--symmetric", 1, "--budget", 0.25, "--partition", 256)
+        assert np.array_equal(_values(tmp_path, np.uint32, og.n), O.bfs(og, src))
+    _run(cli, tmp_path, "--input", path, "--algo", "cc", "--symmetric", 1, "--budget", 0.25)
+    assert np.array_equal(_values(tmp_path, np.uint32, og.n), O.cc(og))

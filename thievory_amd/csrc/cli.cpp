@@ -9,6 +9,9 @@
 //   --partition E     edges per partition (default 65536)
 //   --tol T --maxIter I   PageRank stop rule (default 1e-7, 100)
 //   --autotune FILE   pick the launch shape (cached in FILE)
+//   --symmetric 0|1   the file holds every edge both ways (undirected
+//                     input): BFS may run bottom-up levels, CC runs by
+//                     union-find; results are the same (default 0)
 // Output: the reference's stdout lines (graph.cuh:712-713, bfs/bfs.cu:
 // 527-538) and results/values.bin (n x u32, or n x f64 for PageRank,
 // graph.cuh:817-852; skipped silently when results/ does not exist).
@@ -35,6 +38,7 @@ struct Options {
   unsigned partition = 1u << 16;
   float tol = 1e-7f;
   int max_iter = 100;
+  bool symmetric = false;
 };
 
 void usage(const char *prog) {
@@ -50,7 +54,9 @@ void usage(const char *prog) {
             << "  --gpus : Specify the number of NEIGHBOR GPUs {0} (default = 0)" << std::endl
             << "  --budget : Fraction of edge bytes kept in HBM (default = 90% of free)"
             << std::endl
-            << "  --partition : Edges per partition (default = 65536)" << std::endl;
+            << "  --partition : Edges per partition (default = 65536)" << std::endl
+            << "  --symmetric : The graph holds every edge both ways {0, 1} (default = 0)"
+            << std::endl;
   exit(0);
 }
 
@@ -66,6 +72,9 @@ thv_graph *open_graph(const Options &o, int fmt, int edge_bytes) {
   uint64_t n = 0, m = 0;
   thv_graph_info(g, &n, &m, nullptr, nullptr);
   std::cout << "Num Vertices: " << n << std::endl << "Num Edges: " << m << std::endl;
+  if (o.symmetric && fmt != THV_FMT_CSC &&
+      (rc = thv_graph_set_option(g, THV_OPT_SYMMETRIC, 1.0)))
+    die(rc, "cannot mark the graph symmetric");
   if (o.budget >= 0.0) {
     const uint64_t b = (uint64_t)(o.budget * (double)m * edge_bytes);
     if ((rc = thv_store_config(g, b, o.partition))) die(rc, "store configuration failed");
@@ -201,6 +210,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "--tol")) o.tol = (float)atof(v);
     else if (!strcmp(k, "--maxIter")) o.max_iter = atoi(v);
     else if (!strcmp(k, "--autotune")) o.tune = v;
+    else if (!strcmp(k, "--symmetric")) o.symmetric = atoi(v) != 0;
   }
   if (!has_input) exit(0);  // main.cu:116
   char arch[64] = "";
