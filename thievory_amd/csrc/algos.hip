@@ -517,7 +517,6 @@ void run_cc(Graph &g, uint32_t *out, thv_stats *st) {
 static void sssp_delta(Graph &g, thv_stats *st) {
   Workspace &w = g.ws;
   hipStream_t s = g.stream;
-  const uint64_t words = bitmap_words(g.n);
   ExpandArgs a = base_args(g);
   LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
   uint32_t *pend = w.front[0].as<uint32_t>();
@@ -565,7 +564,6 @@ static void sssp_delta(Graph &g, thv_stats *st) {
     st->expand_ms = expand_ms;
     st->expand_launches = launches;
   }
-  (void)words;
 }
 
 void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
