@@ -44,6 +44,58 @@ def test_batch_64_matches_single_source(g14, algo, frac):
         assert (out["hash"][i], out["reached"][i], out["depth"][i]) == (h, r, d), (i, s)
 
 
+@pytest.mark.parametrize("weights", [True, False])
+@pytest.mark.parametrize("frac", [1.0, 0.25, 0.0])
+def test_batch_bfs_bottom_up_levels(weights, frac):
+    """On a symmetric graph the dense levels of a 64-lane BFS run bottom-up;
+    every lane's levels equal the oracle's and the top-down-only batch's,
+    with the lists in HBM, split, or all read zero-copy."""
+    _say("case: batch bottom-up", weights, frac)
+    og = O.rmat(14, 16, seed=33, weights=weights)
+    g = T.Graph.rmat(14, 16, seed=33, weights=weights)
+    g.store_config(int(frac * g.m * g.edge_bytes), 1 << 10)
+    rng = np.random.default_rng(6)
+    srcs = rng.integers(0, og.n, 64).astype(np.uint32)
+    srcs[0] = 0                        # the highest-degree vertex
+    srcs[5] = srcs[9]
+    deg = np.diff(og.off)
+    srcs[11] = int(np.flatnonzero(deg == 0)[0])   # an isolated source
+    bu = g.batch("bfs", srcs, full=True)
+    assert bu["stats"].bottom_up_levels > 0
+    for i, s in enumerate(srcs):
+        val, h, r, d = _expect(og, "bfs", int(s))
+        assert np.array_equal(bu["full"][i], val), (i, s)
+        assert (bu["hash"][i], bu["reached"][i], bu["depth"][i]) == (h, r, d), (i, s)
+    g.set_option("direction", 0)
+    td = g.batch("bfs", srcs)
+    g.set_option("direction", 1)
+    assert td["stats"].bottom_up_levels == 0
+    assert np.array_equal(td["hash"], bu["hash"])
+    assert np.array_equal(td["depth"], bu["depth"])
+    # early exit: far fewer records than the top-down batch reads
+    assert bu["stats"].edges_traversed < td["stats"].edges_traversed
+
+
+def test_serving_bottom_up_staggered():
+    """A server on a symmetric graph admits queries between levels, so its
+    lanes run at different depths; levels may still run bottom-up, and each
+    result equals the query alone."""
+    _say("case: serving bottom-up")
+    og = O.rmat(13, 16, seed=8)
+    g = T.Graph.from_arrays(og.off, og.adj)
+    g.set_option("symmetric", 1)
+    srv = T.Server(g, max_lanes=16, keep_full=True)
+    rng = np.random.default_rng(12)
+    srcs = [int(s) for s in rng.integers(0, og.n, 120)]
+    tk = [(s, srv.submit("bfs", s)) for s in srcs]
+    srv.drain()
+    for s, t in tk:
+        full = np.zeros(og.n, np.uint32)
+        r = srv.poll(t, full)
+        assert r["status"] == 0 and np.array_equal(full, O.bfs(og, s)), s
+    srv.stop()
+
+
 @pytest.mark.parametrize("k", [1, 13])
 def test_batch_small_k(g14, k):
     g, og = g14

@@ -15,7 +15,14 @@ namespace {
 
 constexpr int MBS = 256;
 constexpr int ACC_HASH = 0, ACC_CNT = 64, ACC_LM = 128, RES_HASH = 129, RES_CNT = 193,
-              RES_DEPTH = 257, ACC_WORDS = 384;
+              RES_DEPTH = 257, ACC_LONG = 321, ACC_MU = 322, ACC_SCAN_HOST = 323,
+              ACC_SCAN_HBM = 324, ACC_WORDS = 384;
+// bottom-up levels: lists up to this length are scanned by one lane, longer
+// ones by a group of BU_G lanes reading BU_R records each per step
+constexpr uint32_t BU_SHORT = 32, BU_G = 16, BU_R = 2;
+// per-wave LDS buffer of long-list vertices, flushed to the global queue with
+// one atomic per flush (an atomic per wave step on one counter serialises)
+constexpr uint32_t BU_LQ = 512;
 
 struct MsArgs {
   const uint32_t *qv[2];
@@ -131,6 +138,296 @@ __global__ void __launch_bounds__(MBS) k_ms_bfs(MsArgs a) {
     atomicAdd(&a.acc[ACC_CNT + tid], (unsigned long long)s_cnt[tid]);
   }
   if (tid == 0 && s_lm) atomicOr(&a.acc[ACC_LM], s_lm);
+}
+
+// ---- bottom-up BFS levels (symmetric graphs) ---------------------------------
+//
+// Every vertex w that some active lane has not reached looks for those lanes
+// among its neighbours' frontier masks: N[w] = (OR of F[v] over the
+// neighbours v) & ~seen[w], and the scan stops as soon as all of them are
+// found.  On a symmetric graph w's out-list is its in-list, so the lanes
+// found are exactly those a top-down level hands w.  Each w has one owner
+// (a lane, or a group for a long list), so N, seen and the levels need no
+// atomics; in the dense middle levels most vertices find all their missing
+// lanes within a few records.
+struct MsBuArgs {
+  const uint64_t *eptr;
+  const uint32_t *deg;
+  const uint32_t *rbit;
+  const uint32_t *fb;           // frontier bitmap: F[v] is read only when v's bit is set
+  const unsigned long long *F;
+  unsigned long long *N, *seen;
+  uint32_t *nb;
+  uint32_t *full;
+  unsigned long long *acc;
+  uint32_t *lq;                 // vertices with long lists, for k_ms_bu_long
+  uint64_t n;
+  unsigned long long act;       // active lanes
+  uint32_t level;
+  uint32_t uval;                // the level value when every active lane shares it, else 0
+  uint32_t voff[64];
+};
+
+// Per-block digest state for bottom-up levels: lane q of every wave keeps
+// query q's sum and count in registers.
+struct BuDigest {
+  unsigned long long h = 0, scan_host = 0, scan_hbm = 0, mu = 0, lm = 0;
+  uint32_t cnt = 0;
+};
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long x, int i) {
+  return ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), i)
+          << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, i);
+}
+
+// The wave's gains, one vertex per set bit of `who` (a lane mask): lane q
+// adds query q's pair (w, value) for each vertex whose gain holds q.  When
+// all active lanes share one value (uval != 0), the vertex's owner lane has
+// hashed the pair already (hw) and the other lanes only add it.
+__device__ __forceinline__ void bu_digest_wave(unsigned long long who, unsigned long long gain,
+                                               uint32_t w, unsigned long long hw, uint32_t uval,
+                                               uint32_t myval, uint32_t lane, uint32_t *full,
+                                               uint64_t n, BuDigest &d) {
+  while (who) {
+    const int i = __builtin_ctzll(who);
+    who &= who - 1;
+    const unsigned long long gi = readlane64(gain, i);
+    d.lm |= gi;
+    if (uval) {
+      const unsigned long long hi = readlane64(hw, i);
+      if ((gi >> lane) & 1ull) {
+        d.h += hi;
+        d.cnt++;
+        if (full) full[(uint64_t)lane * n + (uint32_t)__builtin_amdgcn_readlane((int)w, i)] = uval;
+      }
+    } else {
+      const uint32_t wi = (uint32_t)__builtin_amdgcn_readlane((int)w, i);
+      if ((gi >> lane) & 1ull) {
+        d.h += mix64(((unsigned long long)wi << 32) | myval);
+        d.cnt++;
+        if (full) full[(uint64_t)lane * n + wi] = myval;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void bu_digest_flush(BuDigest &d, unsigned long long *acc,
+                                                unsigned long long *s_hash, unsigned int *s_cnt,
+                                                unsigned long long *s_misc) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  if (d.cnt) {
+    atomicAdd(&s_hash[lane], d.h);
+    atomicAdd(&s_cnt[lane], d.cnt);
+  }
+  if (d.lm) atomicOr(&s_misc[0], d.lm);
+  if (d.mu) atomicAdd(&s_misc[1], d.mu);
+  if (d.scan_host) atomicAdd(&s_misc[2], d.scan_host);
+  if (d.scan_hbm) atomicAdd(&s_misc[3], d.scan_hbm);
+  __syncthreads();
+  if (tid < 64 && s_cnt[tid]) {
+    atomicAdd(&acc[ACC_HASH + tid], s_hash[tid]);
+    atomicAdd(&acc[ACC_CNT + tid], (unsigned long long)s_cnt[tid]);
+  }
+  if (tid == 0) {
+    if (s_misc[0]) atomicOr(&acc[ACC_LM], s_misc[0]);
+    if (s_misc[1]) atomicAdd(&acc[ACC_MU], s_misc[1]);
+    if (s_misc[2]) atomicAdd(&acc[ACC_SCAN_HOST], s_misc[2]);
+    if (s_misc[3]) atomicAdd(&acc[ACC_SCAN_HBM], s_misc[3]);
+  }
+}
+
+// Copy a wave's LDS buffer of `nl` long-list vertices to the global queue.
+__device__ __forceinline__ void bu_lq_flush(const uint32_t *buf, uint32_t nl,
+                                            unsigned long long *acc, uint32_t *lq,
+                                            uint32_t lane) {
+  // the buffer was written by other lanes of this wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint32_t base = 0;
+  if (lane == 0) base = (uint32_t)atomicAdd(&acc[ACC_LONG], (unsigned long long)nl);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+  for (uint32_t i = lane; i < nl; i += 64) lq[base + i] = buf[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sweep: one wave per 64 consecutive vertices.  Lists of up to BU_SHORT
+// records are scanned by their own lane (4 records and 4 frontier masks in
+// flight per step); longer ones are queued for k_ms_bu_long.  The next-
+// frontier bitmap words come from a ballot, one plain store per half wave.
+template <int REC>
+__global__ void __launch_bounds__(MBS) k_ms_bu(MsBuArgs a) {
+  __shared__ unsigned long long s_hash[64];
+  __shared__ unsigned int s_cnt[64];
+  __shared__ unsigned long long s_misc[4];
+  __shared__ uint32_t s_lq[MBS / 64][BU_LQ];
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  uint32_t *my_lq = s_lq[tid >> 6];
+  uint32_t nl = 0;  // entries in my_lq (wave-uniform)
+  if (tid < 64) {
+    s_hash[tid] = 0;
+    s_cnt[tid] = 0;
+  }
+  if (tid < 4) s_misc[tid] = 0;
+  __syncthreads();
+  const uint32_t myval = a.level + 2 - a.voff[lane];
+  BuDigest dg;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (MBS / 64);
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * (MBS / 64) + (tid >> 6)) * 64; w0 < a.n;
+       w0 += nwaves * 64) {
+    const uint64_t w = w0 + lane;
+    uint32_t d = 0;
+    unsigned long long sw = 0, U = 0;
+    if (w < a.n) {
+      d = a.deg[w];
+      sw = a.seen[w];
+    }
+    if (d) U = a.act & ~sw;
+    const bool lng = U && d > BU_SHORT;
+    unsigned long long acc = 0;
+    uint32_t scanned = 0;
+    if (U && !lng) {
+      const unsigned long long b = a.eptr[w];
+      for (uint32_t k = 0; k < d; k += 4) {
+        uint32_t v[4];
+        unsigned long long f[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          v[j] = 0xFFFFFFFFu;
+          if (k + j < d) v[j] = *reinterpret_cast<const uint32_t *>(b + (uint64_t)(k + j) * REC);
+        }
+        uint32_t fw[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          fw[j] = 0;
+          if (v[j] != 0xFFFFFFFFu) fw[j] = a.fb[v[j] >> 5];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          f[j] = 0;
+          if ((fw[j] >> (v[j] & 31)) & 1u) f[j] = a.F[v[j]];
+        }
+        acc |= f[0] | f[1] | f[2] | f[3];
+        scanned = min(d, k + 4);
+        if ((acc & U) == U) break;
+      }
+    }
+    // queue the long lists through the wave's LDS buffer
+    const unsigned long long lb = __ballot(lng);
+    if (lb) {
+      if (lng) my_lq[nl + __popcll(lb & ((1ull << lane) - 1))] = (uint32_t)w;
+      nl += (uint32_t)__popcll(lb);
+      if (nl > BU_LQ - 64) {
+        bu_lq_flush(my_lq, nl, a.acc, a.lq, lane);
+        nl = 0;
+      }
+    }
+    const unsigned long long gain = acc & U;
+    if (gain) {
+      a.N[w] = gain;
+      a.seen[w] = sw | gain;
+    }
+    if (!lng && (U & ~gain)) dg.mu += d;
+    if (scanned) {
+      const bool hbm = (a.rbit[w >> 5] >> (w & 31)) & 1u;
+      if (hbm) dg.scan_hbm += scanned;
+      else dg.scan_host += scanned;
+    }
+    const unsigned long long gb = __ballot(gain != 0);
+    if (lane == 0 && (uint32_t)gb) a.nb[w0 >> 5] = (uint32_t)gb;
+    if (lane == 32 && (gb >> 32)) a.nb[(w0 >> 5) + 1] = (uint32_t)(gb >> 32);
+    unsigned long long hw = 0;
+    if (a.uval && gain) hw = mix64((w << 32) | a.uval);
+    bu_digest_wave(gb, gain, (uint32_t)w, hw, a.uval, myval, lane, a.full, a.n, dg);
+  }
+  if (nl) bu_lq_flush(my_lq, nl, a.acc, a.lq, lane);
+  bu_digest_flush(dg, a.acc, s_hash, s_cnt, s_misc);
+}
+
+// Long lists: a group of BU_G lanes per queued vertex reads BU_G * BU_R
+// records per step, ORs their frontier masks across the group and stops
+// when every missing lane is found.
+template <int REC>
+__global__ void __launch_bounds__(MBS) k_ms_bu_long(MsBuArgs a) {
+  __shared__ unsigned long long s_hash[64];
+  __shared__ unsigned int s_cnt[64];
+  __shared__ unsigned long long s_misc[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, sub = tid & (BU_G - 1);
+  if (tid < 64) {
+    s_hash[tid] = 0;
+    s_cnt[tid] = 0;
+  }
+  if (tid < 4) s_misc[tid] = 0;
+  __syncthreads();
+  const uint32_t myval = a.level + 2 - a.voff[lane];
+  const uint64_t total = a.acc[ACC_LONG];
+  constexpr uint32_t GPW = 64 / BU_G;  // groups per wave
+  BuDigest dg;
+  const uint64_t wave = (uint64_t)blockIdx.x * (MBS / 64) + (tid >> 6);
+  const uint64_t nwaves = (uint64_t)gridDim.x * (MBS / 64);
+  for (uint64_t i0 = wave * GPW; i0 < total; i0 += nwaves * GPW) {
+    const uint64_t i = i0 + (lane / BU_G);
+    uint32_t w = 0;
+    unsigned long long U = 0, sw = 0, acc = 0;
+    uint32_t d = 0, scanned = 0;
+    if (i < total) {
+      w = a.lq[i];
+      d = a.deg[w];
+      sw = a.seen[w];
+      U = a.act & ~sw;
+      const unsigned long long b = a.eptr[w];
+      for (uint32_t k = 0; k < d; k += BU_G * BU_R) {
+        uint32_t v[BU_R];
+        unsigned long long f[BU_R];
+#pragma unroll
+        for (uint32_t r = 0; r < BU_R; r++) {
+          const uint32_t e = k + r * BU_G + sub;
+          v[r] = 0xFFFFFFFFu;
+          if (e < d) v[r] = *reinterpret_cast<const uint32_t *>(b + (uint64_t)e * REC);
+        }
+        uint32_t fw[BU_R];
+#pragma unroll
+        for (uint32_t r = 0; r < BU_R; r++) {
+          fw[r] = 0;
+          if (v[r] != 0xFFFFFFFFu) fw[r] = a.fb[v[r] >> 5];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < BU_R; r++) {
+          f[r] = 0;
+          if ((fw[r] >> (v[r] & 31)) & 1u) f[r] = a.F[v[r]];
+        }
+        unsigned long long x = f[0];
+#pragma unroll
+        for (uint32_t r = 1; r < BU_R; r++) x |= f[r];
+#pragma unroll
+        for (uint32_t o = BU_G / 2; o; o >>= 1) x |= __shfl_xor(x, o, BU_G);
+        acc |= x;
+        scanned = min(d, k + BU_G * BU_R);
+        if ((acc & U) == U) break;
+      }
+    }
+    const bool leader = sub == 0 && i < total;
+    const unsigned long long gain = acc & U;
+    if (leader) {
+      if (gain) {
+        a.N[w] = gain;
+        a.seen[w] = sw | gain;
+        atomicOr(&a.nb[w >> 5], 1u << (w & 31));
+      }
+      if (U & ~gain) dg.mu += d;
+      const bool hbm = (a.rbit[w >> 5] >> (w & 31)) & 1u;
+      if (hbm) dg.scan_hbm += scanned;
+      else dg.scan_host += scanned;
+    }
+    unsigned long long hw = 0;
+    if (a.uval && leader && gain) hw = mix64(((unsigned long long)w << 32) | a.uval);
+    bu_digest_wave(__ballot(leader && gain != 0), gain, w, hw, a.uval, myval, lane, a.full, a.n,
+                   dg);
+  }
+  bu_digest_flush(dg, a.acc, s_hash, s_cnt, s_misc);
 }
 
 // SSSP: lane q of a wave relaxes query q.  A wave owns 64 consecutive
@@ -312,6 +609,7 @@ MultiEngine::MultiEngine(Graph &gr, int al, uint32_t lanes, bool kf)
   if (algo == THV_ALGO_SSSP && g.rec != 8)
     fail(THV_ENOTSUP, "SSSP needs a weighted graph (8-byte records)");
   memset(&stats, 0, sizeof stats);
+  mu_est = g.m;
   store_ensure(g);
   HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   const uint64_t n = g.n, words = bitmap_words(n);
@@ -351,6 +649,7 @@ uint64_t MultiEngine::bytes_needed(const Graph &g, int algo, uint32_t lanes, boo
   const uint64_t n = g.n, words = bitmap_words(n);
   uint64_t b = 2 * (words * 4 + (n + 1) * 8);                       // fb, F
   b += 2 * ((n + 1) * 12) + 2 * (g.m / 1024 + 2) * 4;                // queues
+  b += (n / TILE + 1) * TILE * 4;                                     // long-list queue
   if (algo == THV_ALGO_BFS) b += (n + 1) * 8 + (keep_full ? (uint64_t)lanes * n * 4 : 0);
   else b += (uint64_t)lanes * n * 4 + (keep_full ? n * 4 : 0);
   return b;
@@ -414,13 +713,86 @@ void MultiEngine::launch_level() {
                        seen.as<unsigned long long>(), dist.as<uint32_t>(), L,
                        full.as<uint32_t>(), n, acc.as<unsigned long long>());
     admit_q.clear();
+    mu_est = g.m;  // new lanes miss every vertex
   }
+  level_bu = false;
   if (active) {
     HIPC(hipMemsetAsync(F[cur ^ 1].p, 0, (n + 1) * 8, stream));
     HIPC(hipMemsetAsync(fb[cur ^ 1].p, 0, words * 4, stream));
     HIPC(hipMemsetAsync(acc.as<unsigned long long>() + ACC_LM, 0, 8, stream));
     LevelCtr *c = qs.ctr.as<LevelCtr>() + (level & 1);
-    launch_compact(g, qs, fb[cur].as<uint32_t>(), c, stream);
+    if (algo == THV_ALGO_BFS && g.sym && g.dir_opt) {
+      // direction: the frontier's edges against the edges of the vertices
+      // some lane still misses (an upper bound kept since the last
+      // bottom-up level; any admission resets it to m)
+      launch_compact(g, qs, fb[cur].as<uint32_t>(), c, stream, CMP_COUNT_ONLY);
+      HIPC(hipMemcpyAsync(&h_ctr[level & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost,
+                          stream));
+      HIPC(hipStreamSynchronize(stream));
+      const uint64_t mf = h_ctr[level & 1].qw[0] + h_ctr[level & 1].qw[1];
+      level_bu = BU_ALPHA * mf > mu_est && mf > n / 8;
+      if (level_bu) {
+        launch_bottom_up();
+      } else {
+        launch_compact(g, qs, fb[cur].as<uint32_t>(), c, stream, CMP_WRITE_ONLY);
+        launch_top_down(c);
+      }
+    } else {
+      launch_compact(g, qs, fb[cur].as<uint32_t>(), c, stream);
+      launch_top_down(c);
+    }
+    launched_expand = true;
+  }
+  HIPC(hipMemcpyAsync(h_acc, acc.p, ACC_WORDS * 8, hipMemcpyDeviceToHost, stream));
+}
+
+void MultiEngine::launch_bottom_up() {
+  const uint64_t n = g.n;
+  HIPC(hipMemsetAsync(acc.as<unsigned long long>() + ACC_LONG, 0, 4 * 8, stream));
+  MsBuArgs b;
+  memset(&b, 0, sizeof b);
+  b.eptr = g.store.d_eptr.as<uint64_t>();
+  b.deg = g.d_deg.as<uint32_t>();
+  b.rbit = g.store.d_rbit.as<uint32_t>();
+  b.fb = fb[cur].as<uint32_t>();
+  b.F = F[cur].as<unsigned long long>();
+  b.N = F[cur ^ 1].as<unsigned long long>();
+  b.seen = seen.as<unsigned long long>();
+  b.nb = fb[cur ^ 1].as<uint32_t>();
+  b.full = keep_full ? full.as<uint32_t>() : nullptr;
+  b.acc = acc.as<unsigned long long>();
+  b.lq = qs.slist.as<uint32_t>();
+  b.n = n;
+  b.act = active;
+  b.level = level;
+  for (uint32_t q = 0; q < MAXL; q++) b.voff[q] = voff[q];
+  // one value for all active lanes (a cohort admitted together): hash each
+  // gained vertex once
+  b.uval = 0;
+  for (uint32_t q = 0; q < MAXL; q++) {
+    if (!((active >> q) & 1ull)) continue;
+    const uint32_t v = level + 2 - voff[q];
+    if (!b.uval) b.uval = v;
+    else if (b.uval != v) {
+      b.uval = 0;
+      break;
+    }
+  }
+  if (qs.slist.bytes < n * 4) fail(THV_ESTATE, "bottom-up queue too small");
+  const int gs = grid_for(n, 8 * g.cus), gl = 8 * g.cus;
+  if (g.rec == 8) {
+    hipLaunchKernelGGL(k_ms_bu<8>, dim3(gs), dim3(MBS), 0, stream, b);
+    hipLaunchKernelGGL(k_ms_bu_long<8>, dim3(gl), dim3(MBS), 0, stream, b);
+  } else {
+    hipLaunchKernelGGL(k_ms_bu<4>, dim3(gs), dim3(MBS), 0, stream, b);
+    hipLaunchKernelGGL(k_ms_bu_long<4>, dim3(gl), dim3(MBS), 0, stream, b);
+  }
+  HIPC(hipGetLastError());
+}
+
+void MultiEngine::launch_top_down(LevelCtr *c) {
+  const uint64_t n = g.n;
+  {
     MsArgs a;
     memset(&a, 0, sizeof a);
     for (int i = 0; i < 2; i++) {
@@ -458,9 +830,7 @@ void MultiEngine::launch_level() {
     }
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(&h_ctr[level & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, stream));
-    launched_expand = true;
   }
-  HIPC(hipMemcpyAsync(h_acc, acc.p, ACC_WORDS * 8, hipMemcpyDeviceToHost, stream));
 }
 
 void MultiEngine::finish_level(std::vector<LaneResult> &done) {
@@ -482,10 +852,19 @@ void MultiEngine::finish_level(std::vector<LaneResult> &done) {
   }
   just_finished = 0;
   if (!launched_expand) return;
-  const LevelCtr &h = h_ctr[level & 1];
-  stats.edges_traversed += h.qw[0] + h.qw[1];
-  stats.host_bytes += h.qw[0] * (uint64_t)g.rec;
-  stats.hbm_bytes += h.qw[1] * (uint64_t)g.rec;
+  if (level_bu) {
+    const uint64_t sh = h_acc[ACC_SCAN_HOST], sd = h_acc[ACC_SCAN_HBM];
+    stats.edges_traversed += sh + sd;
+    stats.host_bytes += sh * (uint64_t)g.rec;
+    stats.hbm_bytes += sd * (uint64_t)g.rec;
+    stats.bottom_up_levels++;
+    mu_est = h_acc[ACC_MU];
+  } else {
+    const LevelCtr &h = h_ctr[level & 1];
+    stats.edges_traversed += h.qw[0] + h.qw[1];
+    stats.host_bytes += h.qw[0] * (uint64_t)g.rec;
+    stats.hbm_bytes += h.qw[1] * (uint64_t)g.rec;
+  }
   stats.iterations++;
   stats.expand_launches++;
   const uint64_t lm = h_acc[ACC_LM];

@@ -5,7 +5,9 @@
 //         (lanes with v in their frontier), N[v] (next frontier).  An edge
 //         v -> w hands w the lanes F[v] & ~seen[w] (atomicOr); each newly
 //         reached (lane, vertex) pair adds mix64(w << 32 | value) to the
-//         lane's digest, so no per-lane value array is needed.
+//         lane's digest, so no per-lane value array is needed.  On a
+//         symmetric graph dense levels run bottom-up: each vertex ORs its
+//         neighbours' F masks until it has every lane it misses.
 //   SSSP: lane-packed distances dist[v * L + q]; one wave relaxes one edge
 //         for all its active lanes at once (lane q = query q), and the ballot
 //         of improved lanes becomes w's next-frontier mask.
@@ -61,6 +63,14 @@ struct MultiEngine {
   std::vector<std::pair<uint32_t, uint32_t>> admit_q;  // (lane, src)
   bool launched_expand = false;
   bool launched_finish = false;
+  // direction of BFS levels on a symmetric graph: bottom-up when
+  // BU_ALPHA x (frontier edges) exceeds mu_est, an upper bound on the edges
+  // of the vertices some active lane has not reached (measured by the last
+  // bottom-up level, reset to m by every admission), and the frontier holds
+  // more than n/8 edges (a bottom-up level sweeps all n vertices)
+  static constexpr uint64_t BU_ALPHA = 2;
+  uint64_t mu_est = 0;
+  bool level_bu = false;
 
   MultiEngine(Graph &g, int algo, uint32_t lanes, bool keep_full);
   ~MultiEngine();
@@ -74,6 +84,8 @@ struct MultiEngine {
   int admit(uint32_t src, uint64_t tag);   // -> lane, or -1 when none is free
   bool busy() const { return active || pend_finish || pend_clear || !admit_q.empty(); }
   void launch_level();                     // enqueue one level on `stream`
+  void launch_top_down(LevelCtr *c);       // expansion of the compacted frontier
+  void launch_bottom_up();                 // sweep + long lists (BFS, symmetric)
   void finish_level(std::vector<LaneResult> &done);  // sync, harvest finished lanes
   void copy_full(uint32_t lane, uint32_t *out);      // value array of a lane (keep_full)
 };

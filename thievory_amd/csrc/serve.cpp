@@ -1,13 +1,21 @@
 // serve.cpp -- continuous-batching query server (no reference counterpart).
 //
-// One server thread owns two multi-source engines (multi.hpp), BFS and
-// SSSP, each on its own HIP stream so their levels overlap on the device.
+// One server thread owns multi-source engines (multi.hpp) for BFS and SSSP,
+// each on its own HIP stream so their levels overlap on the device.
 // Between levels it admits waiting queries into free lanes (a lane is the
 // unit of frontier/scratch memory, so admission is bounded by the memory the
 // engines were sized for), launches one level of each busy engine, then
-// harvests the lanes that finished.  Submit and poll are thread-safe; the
-// graph stays resident for the server's lifetime (store reconfiguration is
-// refused meanwhile).
+// harvests the lanes that finished.
+//
+// BFS on a symmetric graph (direction-optimising) is served in cohorts: a
+// BFS engine admits only when none of its lanes is running, so its lanes
+// stay at one depth and its dense levels can run bottom-up with early exit
+// (lanes at other depths would keep every vertex "missing").  Two BFS
+// engines alternate, so a new cohort starts while the previous one is
+// still running.  Elsewhere admission is continuous.
+//
+// Submit and poll are thread-safe; the graph stays resident for the
+// server's lifetime (store reconfiguration is refused meanwhile).
 #include <condition_variable>
 #include <deque>
 #include <thread>
@@ -21,7 +29,8 @@ void set_last_error(const std::string &m);
 struct Server {
   Graph *g = nullptr;
   thv_serve_cfg cfg{};
-  std::unique_ptr<MultiEngine> eng[2];  // [0] BFS, [1] SSSP
+  std::vector<std::unique_ptr<MultiEngine>> eng;  // BFS engine(s), then SSSP
+  bool cohort = false;                  // BFS admits only into idle engines
   std::thread th;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -63,39 +72,41 @@ void Server::loop() {
       {
         std::unique_lock<std::mutex> lk(mu);
         auto idle = [&] {
-          for (int a = 0; a < 2; a++)
-            if ((eng[a] && eng[a]->busy()) || !waiting[a].empty()) return false;
-          return true;
+          for (auto &e : eng)
+            if (e->busy()) return false;
+          return waiting[0].empty() && waiting[1].empty();
         };
         cv_work.wait(lk, [&] { return stop || !idle(); });
         if (stop && idle()) break;
-        for (int a = 0; a < 2; a++) {
-          if (!eng[a]) continue;
-          while (!waiting[a].empty() && eng[a]->free_lanes()) {
+        for (auto &e : eng) {
+          const int a = e->algo == THV_ALGO_BFS ? 0 : 1;
+          if (a == 0 && cohort && e->active) continue;
+          while (!waiting[a].empty() && e->free_lanes()) {
             const Req r = waiting[a].front();
             waiting[a].pop_front();
-            eng[a]->admit(r.src, r.ticket);
+            e->admit(r.src, r.ticket);
             res[r.ticket].start_ms = now_ms();
           }
         }
       }
-      bool launched[2] = {false, false};
-      for (int a = 0; a < 2; a++)
-        if (eng[a] && eng[a]->busy()) {
-          eng[a]->launch_level();
-          launched[a] = true;
+      std::vector<char> launched(eng.size(), 0);
+      for (size_t i = 0; i < eng.size(); i++)
+        if (eng[i]->busy()) {
+          eng[i]->launch_level();
+          launched[i] = 1;
         }
       done.clear();
-      for (int a = 0; a < 2; a++) {
-        if (!launched[a]) continue;
-        eng[a]->finish_level(done);
-        if (cfg.keep_full && eng[a]->just_finished) {
-          for (uint32_t q = 0; q < eng[a]->L; q++) {
-            if (!((eng[a]->just_finished >> q) & 1ull)) continue;
+      for (size_t i = 0; i < eng.size(); i++) {
+        if (!launched[i]) continue;
+        MultiEngine &e = *eng[i];
+        e.finish_level(done);
+        if (cfg.keep_full && e.just_finished) {
+          for (uint32_t q = 0; q < e.L; q++) {
+            if (!((e.just_finished >> q) & 1ull)) continue;
             std::vector<uint32_t> v(g->n);
-            eng[a]->copy_full(q, v.data());
+            e.copy_full(q, v.data());
             std::lock_guard<std::mutex> lk(mu);
-            full[eng[a]->tag_of[q]] = std::move(v);
+            full[e.tag_of[q]] = std::move(v);
           }
         }
       }
@@ -160,14 +171,19 @@ int thv_serve_start(thv_graph *gh, const thv_serve_cfg *cfg, thv_server **out) {
     const bool kf = cfg->keep_full != 0;
     // BFS lanes first (cheap: 3 masks per vertex), SSSP lanes get the rest,
     // halving until the lane-packed distances fit the scratch budget.
-    const uint64_t need_bfs = MultiEngine::bytes_needed(*g, THV_ALGO_BFS, lanes, kf);
+    uint64_t need_bfs = MultiEngine::bytes_needed(*g, THV_ALGO_BFS, lanes, kf);
     if (need_bfs > budget)
       fail(THV_ENOMEM, "scratch budget too small for the BFS engine");
-    s->eng[0].reset(new MultiEngine(*g, THV_ALGO_BFS, lanes, kf));
+    s->cohort = g->sym && g->dir_opt;
+    s->eng.emplace_back(new MultiEngine(*g, THV_ALGO_BFS, lanes, kf));
+    if (s->cohort && 2 * need_bfs <= budget) {
+      s->eng.emplace_back(new MultiEngine(*g, THV_ALGO_BFS, lanes, kf));
+      need_bfs *= 2;
+    }
     if (g->rec == 8) {
       uint32_t sl = lanes;
       while (sl && MultiEngine::bytes_needed(*g, THV_ALGO_SSSP, sl, kf) > budget - need_bfs) sl /= 2;
-      if (sl) s->eng[1].reset(new MultiEngine(*g, THV_ALGO_SSSP, sl, kf));
+      if (sl) s->eng.emplace_back(new MultiEngine(*g, THV_ALGO_SSSP, sl, kf));
     }
     g->servers++;
     s->t0 = now_ns();
@@ -184,7 +200,9 @@ int thv_serve_submit(thv_server *sh, int algo, uint32_t src, uint64_t *ticket) {
     if (!s || !ticket) fail(THV_EINVAL, "null argument");
     const int a = algo == THV_ALGO_BFS ? 0 : algo == THV_ALGO_SSSP ? 1 : -1;
     if (a < 0) fail(THV_EINVAL, "serving takes BFS or SSSP queries");
-    if (!s->eng[a]) fail(THV_ENOTSUP, "no engine for this algorithm (unweighted graph or memory)");
+    bool have = false;
+    for (auto &e : s->eng) have |= e->algo == algo;
+    if (!have) fail(THV_ENOTSUP, "no engine for this algorithm (unweighted graph or memory)");
     if (src >= s->g->n) fail(THV_EINVAL, "source vertex out of range");
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->err) fail(THV_ESTATE, "server failed: " + s->errmsg);
