@@ -18,8 +18,10 @@ constexpr int ACC_HASH = 0, ACC_CNT = 64, ACC_LM = 128, RES_HASH = 129, RES_CNT 
               RES_DEPTH = 257, ACC_LONG = 321, ACC_MU = 322, ACC_SCAN_HOST = 323,
               ACC_SCAN_HBM = 324, ACC_WORDS = 384;
 // bottom-up levels: lists up to this length are scanned by one lane, longer
-// ones by a group of BU_G lanes reading BU_R records each per step
-constexpr uint32_t BU_SHORT = 32, BU_G = 16, BU_R = 2;
+// ones by a group of BU_G lanes reading BU_R records each per step (measured
+// on the 64-lane R-MAT 25 batch: 8 / 8 / 2 beat 32 / 16 / 2 by 5%; larger
+// steps lose more to reads past the early exit than they save in steps)
+constexpr uint32_t BU_SHORT = 8, BU_G = 8, BU_R = 2;
 // per-wave LDS buffer of long-list vertices, flushed to the global queue with
 // one atomic per flush (an atomic per wave step on one counter serialises)
 constexpr uint32_t BU_LQ = 512;
