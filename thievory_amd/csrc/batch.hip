@@ -497,18 +497,34 @@ __global__ void __launch_bounds__(MBS) k_ms_sssp(MsArgs a) {
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) dw[j] = on[j] ? a.dist[(uint64_t)w[j] * L + lane] : 0u;
+        // An improvement is judged against the row just read, so the
+        // atomicMin needs no result and the 8 edges' atomics stay in flight
+        // together.  A lane whose atomicMin loses to a smaller concurrent
+        // write is marked anyway: that write lowered dist[w] in this level
+        // and marks the same (w, lane), so the next frontier is unchanged.
+        unsigned long long b[8], old[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           bool imp = false;
           if (on[j]) {
             const unsigned long long nd = (unsigned long long)dv[j] + wt[j];
-            if (nd < dw[j])
-              imp = nd < atomicMin(&a.dist[(uint64_t)w[j] * L + lane], (uint32_t)nd);
+            if (nd < dw[j]) {
+              imp = true;
+              atomicMin(&a.dist[(uint64_t)w[j] * L + lane], (uint32_t)nd);
+            }
           }
-          const unsigned long long b = __ballot(imp);
-          if (b && lane == 0) {
-            if (atomicOr(&a.N[w[j]], b) == 0ull) atomicOr(&a.nb[w[j] >> 5], 1u << (w[j] & 31));
-            atomicOr(&s_lm, b);
+          b[j] = __ballot(imp);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          old[j] = ~0ull;
+          if (b[j] && lane == 0) old[j] = atomicOr(&a.N[w[j]], b[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          if (b[j] && lane == 0) {
+            if (old[j] == 0ull) atomicOr(&a.nb[w[j] >> 5], 1u << (w[j] & 31));
+            atomicOr(&s_lm, b[j]);
           }
         }
       }
