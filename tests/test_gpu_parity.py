@@ -92,6 +92,26 @@ def rmat16():
     return g, og
 
 
+def test_rmat20_plain_bfs_from_vertex_1():
+    """The plain configuration: BFS from vertex 1 on the undirected R-MAT
+    scale 20 / edge factor 16 graph, every edge in HBM; levels bit-exact,
+    with and without bottom-up levels; CC of the same graph."""
+    _say("case: rmat20 plain bfs")
+    g = T.Graph.rmat(20, 16, seed=1)
+    og = O.rmat(20, 16, seed=1)
+    g.store_config(None, 1 << 16)
+    ref = O.bfs(og, 1)
+    lev, st = g.bfs(1)
+    assert np.array_equal(lev, ref)
+    assert st.bottom_up_levels > 0
+    g.set_option("direction", 0)
+    lev_td, _ = g.bfs(1)
+    g.set_option("direction", 1)
+    assert np.array_equal(lev_td, ref)
+    lab, _ = g.cc()
+    assert np.array_equal(lab, O.cc(og))
+
+
 @pytest.mark.parametrize("frac,part", [(1.0, 1 << 16), (0.25, 1 << 16), (0.25, 1024),
                                        (0.0, 4096)])
 def test_bfs_cc_sssp_split_store(rmat16, frac, part):

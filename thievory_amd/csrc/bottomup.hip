@@ -81,52 +81,6 @@ __global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *k
   }
 }
 
-// One wave per 1024 vertices (32 bitmap words), in three batches of loads:
-// lanes 0-31 read the 32 visited words, every lane then reads the hints of
-// its 16 unvisited vertices, then their frontier bits.  Each word's hits are
-// gathered in the lane that read it, which writes the word once (no
-// atomics: no other kernel runs on these bitmaps meanwhile).
-constexpr uint32_t HSUB = 1024, HROUNDS = HSUB / 64;
-
-__global__ void __launch_bounds__(BBS) k_hint_pass(const uint32_t *hint, const uint32_t *front,
-                                                  uint32_t *vis, uint32_t *next, uint64_t n,
-                                                  unsigned long long *hits) {
-  const uint32_t lane = threadIdx.x & 63;
-  const unsigned long long nwaves = (unsigned long long)gridDim.x * BBS / 64;
-  unsigned long long cnt = 0;
-  for (unsigned long long wv = (blockIdx.x * (unsigned long long)BBS + threadIdx.x) / 64;
-       wv * HSUB < n; wv += nwaves) {
-    const uint64_t base = wv * HSUB, w0 = base >> 5;
-    const bool owner = lane < 32 && ((w0 + lane) << 5) < n;
-    const uint32_t vw = owner ? vis[w0 + lane] : 0xFFFFFFFFu;
-    uint32_t h[HROUNDS];
-#pragma unroll
-    for (uint32_t r = 0; r < HROUNDS; r++) {
-      const uint32_t word = (uint32_t)__shfl((int)vw, (int)(2 * r + (lane >> 5)), 64);
-      const uint64_t v = base + r * 64 + lane;
-      h[r] = (v < n && !((word >> (lane & 31)) & 1u)) ? hint[v] : NO_HINT;
-    }
-    uint32_t f = 0;  // bit r: the hint of round r's vertex is in the frontier
-#pragma unroll
-    for (uint32_t r = 0; r < HROUNDS; r++)
-      if (h[r] != NO_HINT && ((front[h[r] >> 5] >> (h[r] & 31)) & 1u)) f |= 1u << r;
-    uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
-#pragma unroll
-    for (uint32_t r = 0; r < HROUNDS; r++) {
-      const unsigned long long b = __ballot((f >> r) & 1u);
-      if (lane == 2 * r) m = (uint32_t)b;
-      if (lane == 2 * r + 1) m = (uint32_t)(b >> 32);
-    }
-    if (owner && m) {
-      vis[w0 + lane] = vw | m;
-      next[w0 + lane] |= m;
-      cnt += __popc(m);
-    }
-  }
-  cnt = wave_sum(cnt);
-  if (lane == 0 && cnt) atomicAdd(hits, cnt);
-}
-
 }  // namespace
 
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
@@ -162,15 +116,6 @@ void build_parent_hints(Graph &g) {
                      key.as<unsigned long long>(), g.n, g.store.d_hint.as<uint32_t>());
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(s));
-}
-
-void launch_hint_pass(Graph &g, const uint32_t *front, uint32_t *vis, uint32_t *next,
-                      unsigned long long *hits, hipStream_t s) {
-  const uint64_t waves = div_up(g.n, HSUB);
-  const int grid = (int)std::min<uint64_t>(div_up(waves, BBS / 64), 16ull * g.cus);
-  hipLaunchKernelGGL(k_hint_pass, dim3(grid ? grid : 1), dim3(BBS), 0, s,
-                     g.store.d_hint.as<uint32_t>(), front, vis, next, g.n, hits);
-  HIPC(hipGetLastError());
 }
 
 }  // namespace thv

@@ -184,10 +184,15 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
     }
     __syncthreads();
   }
-  if (a.minout) {  // one atomic per wave that saw a selected vertex
+  if (a.minout) {
+    // one atomic per wave that holds a smaller value than the one stored
+    // (all waves hitting one address serialise; the stored minimum falls
+    // fast, so most waves only read it)
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) fmin = min(fmin, (uint32_t)__shfl_xor((int)fmin, d, 64));
-    if (lane == 0 && fmin != 0xFFFFFFFFu) atomicMin(a.minout, fmin);
+    if (lane == 0 && fmin != 0xFFFFFFFFu &&
+        fmin < __hip_atomic_load(a.minout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(a.minout, fmin);
   }
 }
 

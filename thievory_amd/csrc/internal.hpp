@@ -282,10 +282,6 @@ struct BottomUpArgs {
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
 void build_parent_hints(Graph &g);
-// every unvisited vertex whose hint is in front joins vis and next (plain
-// word writes: nothing else may write vis / next meanwhile); hits summed
-void launch_hint_pass(Graph &g, const uint32_t *front, uint32_t *vis, uint32_t *next,
-                      unsigned long long *hits, hipStream_t s);
 ExpandArgs expand_args(const Graph &g, const QueueSet &qs);
 inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);
