@@ -315,6 +315,67 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     const uint64_t sn = ps.qn[0] + ps.qn[1], sr = ps.qw[0] + ps.qw[1];
     probe_bytes += 8 * sn + rec * ps.qw[1] + 4 * sr;
   };
+  // Level bookkeeping, run after the next level's count has synchronised
+  // the stream (one host round trip per level instead of two).
+  struct LevelRec {
+    bool valid = false, bu = false, probed = false, hinted = false;
+    uint32_t L = 0;
+    uint64_t nf = 0, mf = 0;
+    LevelCtr h;
+    double td_cost = 0, bu_cost = 0;
+  } pend;
+  auto finish_level = [&](const LevelRec &p) {
+    if (!p.valid) return;
+    if (p.bu && !p.probed) probe_done();
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
+    memset(&hs, 0, sizeof hs);
+    memset(&hb, 0, sizeof hb);
+    if (p.bu) {
+      hb = w.h_ctr[2];
+      if (D > 0) hs = w.h_ctr[3];
+      bottom_up_ms += ms;
+    } else {
+      expand_ms += ms;
+      launches++;
+    }
+    if (trace) {
+      fprintf(stderr,
+              "[thv] bfs L=%u %s nf=%llu mf=%llu front_host_edges=%llu est_td_ms=%.3f "
+              "est_bu_ms=%.3f hint_hits_total=%llu short=%llu short_host_edges=%llu long=%llu "
+              "long_host=%llu read_host=%llu read_hbm=%llu kernel_ms=%.3f\n",
+              p.L, p.bu ? "bottom-up" : "top-down", (unsigned long long)p.nf,
+              (unsigned long long)p.mf, p.h.qw[0], p.td_cost * 1e3, p.bu_cost * 1e3,
+              p.hinted ? w.h_aux[1] : 0ull,
+              hs.qn[0] + hs.qn[1], hs.qw[0], hb.qn[0] + hb.qn[1], hb.qn[0],
+              p.bu ? hs.qw[0] + hb.rd[0] : p.h.qw[0], p.bu ? hs.qw[1] + hb.rd[1] : p.h.qw[1], ms);
+    }
+    if (st) {
+      st->edges_traversed += p.mf;
+      st->iterations++;
+      st->topdown_host_bytes += p.h.qw[0] * rec;
+      st->topdown_hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + 8 * words;
+      if (!p.bu) {
+        st->edges_examined += p.mf;
+        st->host_bytes += p.h.qw[0] * rec;
+        st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + 8 * words;
+        st->host_partitions_touched += p.h.touched[0];
+      } else {
+        const uint64_t rh = hs.qw[0] + hb.rd[0], rd = hs.qw[1] + hb.rd[1];
+        st->edges_examined += rh + rd;
+        st->host_bytes += rh * rec;
+        st->bottom_up_host_bytes += rh * rec;
+        // early-exit part (the probe's bytes are added at the end): records
+        // in HBM, a frontier word per record, 12 bytes per long unvisited
+        // list, and the visited / frontier bitmaps
+        st->hbm_bytes += hb.rd[1] * rec + 4 * (hb.rd[0] + hb.rd[1]) +
+                         12 * (hb.qn[0] + hb.qn[1]) + 12 * words;
+        st->host_partitions_touched += hb.touched[0] + hs.touched[0];
+        st->bottom_up_levels++;
+      }
+    }
+  };
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
@@ -322,6 +383,8 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
                    w.val.as<uint32_t>(), L + 1);
     HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    finish_level(pend);
+    pend.valid = false;
     const LevelCtr h = w.h_ctr[L & 1];
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
     if (nf == 0) break;
@@ -397,56 +460,16 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       HIPC(hipEventRecord(w.ev[1], s));
       HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     }
-    HIPC(hipStreamSynchronize(s));
-    if (bu && !probed) probe_done();
-    float ms = 0;
-    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
-    LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
-    memset(&hs, 0, sizeof hs);
-    memset(&hb, 0, sizeof hb);
-    if (bu) {
-      hb = w.h_ctr[2];
-      if (D > 0) hs = w.h_ctr[3];
-      bottom_up_ms += ms;
-    } else {
-      expand_ms += ms;
-      launches++;
-    }
-    if (trace) {
-      fprintf(stderr,
-              "[thv] bfs L=%u %s nf=%llu mf=%llu front_host_edges=%llu est_td_ms=%.3f "
-              "est_bu_ms=%.3f hint_hits_total=%llu short=%llu short_host_edges=%llu long=%llu "
-              "long_host=%llu read_host=%llu read_hbm=%llu kernel_ms=%.3f\n",
-              L, bu ? "bottom-up" : "top-down", (unsigned long long)nf,
-              (unsigned long long)mf, h.qw[0], td_cost * 1e3, bu_cost * 1e3,
-              hinted ? w.h_aux[1] : 0ull,
-              hs.qn[0] + hs.qn[1], hs.qw[0], hb.qn[0] + hb.qn[1], hb.qn[0],
-              bu ? hs.qw[0] + hb.rd[0] : h.qw[0], bu ? hs.qw[1] + hb.rd[1] : h.qw[1], ms);
-    }
-    if (st) {
-      st->edges_traversed += mf;
-      st->iterations++;
-      st->topdown_host_bytes += h.qw[0] * rec;
-      st->topdown_hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
-      if (!bu) {
-        st->edges_examined += mf;
-        st->host_bytes += h.qw[0] * rec;
-        st->hbm_bytes += h.qw[1] * rec + 4 * mf + 12 * nf + 8 * words;
-        st->host_partitions_touched += h.touched[0];
-      } else {
-        const uint64_t rh = hs.qw[0] + hb.rd[0], rd = hs.qw[1] + hb.rd[1];
-        st->edges_examined += rh + rd;
-        st->host_bytes += rh * rec;
-        st->bottom_up_host_bytes += rh * rec;
-        // early-exit part (the probe's bytes are added at the end): records
-        // in HBM, a frontier word per record, 12 bytes per long unvisited
-        // list, and the visited / frontier bitmaps
-        st->hbm_bytes += hb.rd[1] * rec + 4 * (hb.rd[0] + hb.rd[1]) +
-                         12 * (hb.qn[0] + hb.qn[1]) + 12 * words;
-        st->host_partitions_touched += hb.touched[0] + hs.touched[0];
-        st->bottom_up_levels++;
-      }
-    }
+    pend.valid = true;
+    pend.bu = bu;
+    pend.probed = probed;
+    pend.hinted = hinted;
+    pend.L = L;
+    pend.nf = nf;
+    pend.mf = mf;
+    pend.h = h;
+    pend.td_cost = td_cost;
+    pend.bu_cost = bu_cost;
     cur ^= 1;
   }
   if (st) {
