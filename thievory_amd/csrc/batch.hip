@@ -483,18 +483,45 @@ __global__ void __launch_bounds__(MBS) k_ms_sssp(MsArgs a) {
         my_w = r.x;
         my_wt = r.y;
       }
+      // consecutive edges mostly share their source: its lane mask and
+      // distance row are read once per source, not once per edge (a stale
+      // row is still a valid distance; an improved source is in N anyway)
+      uint32_t kprev = 0xFFFFFFFFu, dprev = 0;
+      bool mprev = false;
       for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
         uint32_t w[8], dv[8], dw[8], wt[8];
         bool on[8];
+        uint32_t kk[8], ds[8];
+        bool nw[8], ms[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           const uint32_t jj = min(j0 + j, cnt - 1);
           w[j] = __builtin_amdgcn_readlane(my_w, jj);
           wt[j] = __builtin_amdgcn_readlane(my_wt, jj);
-          const uint32_t k = __builtin_amdgcn_readlane(my_k, jj);
-          on[j] = j0 + j < cnt && lane_ok && ((s_m[k] >> lane) & 1ull);
-          dv[j] = on[j] ? a.dist[(uint64_t)s_v[k] * L + lane] : 0u;
+          kk[j] = __builtin_amdgcn_readlane(my_k, jj);
+          nw[j] = kk[j] != (j ? kk[j - 1] : kprev);  // a new source (uniform)
         }
+        // the new sources' lane masks, then their rows, all in flight together
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          ms[j] = false;
+          if (nw[j]) ms[j] = lane_ok && ((s_m[kk[j]] >> lane) & 1ull);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          ds[j] = 0u;
+          if (nw[j] && ms[j]) ds[j] = a.dist[(uint64_t)s_v[kk[j]] * L + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          if (nw[j]) {
+            mprev = ms[j];
+            dprev = ds[j];
+          }
+          on[j] = j0 + j < cnt && mprev;
+          dv[j] = dprev;
+        }
+        kprev = kk[7];
 #pragma unroll
         for (int j = 0; j < 8; j++) dw[j] = on[j] ? a.dist[(uint64_t)w[j] * L + lane] : 0u;
         // An improvement is judged against the row just read, so the
