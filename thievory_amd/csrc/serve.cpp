@@ -1,11 +1,13 @@
 // serve.cpp -- continuous-batching query server (no reference counterpart).
 //
 // One server thread owns multi-source engines (multi.hpp) for BFS and SSSP,
-// each on its own HIP stream so their levels overlap on the device.
-// Between levels it admits waiting queries into free lanes (a lane is the
-// unit of frontier/scratch memory, so admission is bounded by the memory the
-// engines were sized for), launches one level of each busy engine, then
-// harvests the lanes that finished.
+// each on its own HIP stream so their levels overlap on the device.  It
+// launches the next level of every idle busy engine, polls the streams, and
+// for each engine whose level has finished harvests the lanes that ended
+// and admits waiting queries into free lanes (a lane is the unit of
+// frontier/scratch memory, so admission is bounded by the memory the
+// engines were sized for).  Engines advance independently: a BFS cohort's
+// short levels do not wait for a long SSSP level.
 //
 // BFS on a symmetric graph (direction-optimising) is served in cohorts: a
 // BFS engine admits only when none of its lanes is running, so its lanes
@@ -67,49 +69,61 @@ void Server::fail_all(int code, const std::string &msg) {
 
 void Server::loop() {
   std::vector<LaneResult> done;
+  // engines whose level is on the device; each runs at its own pace (a
+  // BFS level is not held back by a long SSSP level on the other stream)
+  std::vector<char> inflight(eng.size(), 0);
   try {
     for (;;) {
       {
         std::unique_lock<std::mutex> lk(mu);
         auto idle = [&] {
-          for (auto &e : eng)
-            if (e->busy()) return false;
+          for (size_t i = 0; i < eng.size(); i++)
+            if (inflight[i] || eng[i]->busy()) return false;
           return waiting[0].empty() && waiting[1].empty();
         };
         cv_work.wait(lk, [&] { return stop || !idle(); });
         if (stop && idle()) break;
-        for (auto &e : eng) {
-          const int a = e->algo == THV_ALGO_BFS ? 0 : 1;
-          if (a == 0 && cohort && e->active) continue;
-          while (!waiting[a].empty() && e->free_lanes()) {
+        // admission between an engine's levels only
+        for (size_t i = 0; i < eng.size(); i++) {
+          MultiEngine &e = *eng[i];
+          if (inflight[i]) continue;
+          const int a = e.algo == THV_ALGO_BFS ? 0 : 1;
+          if (a == 0 && cohort && e.active) continue;
+          while (!waiting[a].empty() && e.free_lanes()) {
             const Req r = waiting[a].front();
             waiting[a].pop_front();
-            e->admit(r.src, r.ticket);
+            e.admit(r.src, r.ticket);
             res[r.ticket].start_ms = now_ms();
           }
         }
       }
-      std::vector<char> launched(eng.size(), 0);
       for (size_t i = 0; i < eng.size(); i++)
-        if (eng[i]->busy()) {
+        if (!inflight[i] && eng[i]->busy()) {
           eng[i]->launch_level();
-          launched[i] = 1;
+          inflight[i] = 1;
         }
       done.clear();
+      bool progressed = false;
       for (size_t i = 0; i < eng.size(); i++) {
-        if (!launched[i]) continue;
+        if (!inflight[i]) continue;
         MultiEngine &e = *eng[i];
+        const hipError_t q = hipStreamQuery(e.stream);
+        if (q == hipErrorNotReady) continue;
+        HIPC(q);
+        inflight[i] = 0;
+        progressed = true;
         e.finish_level(done);
         if (cfg.keep_full && e.just_finished) {
-          for (uint32_t q = 0; q < e.L; q++) {
-            if (!((e.just_finished >> q) & 1ull)) continue;
+          for (uint32_t l = 0; l < e.L; l++) {
+            if (!((e.just_finished >> l) & 1ull)) continue;
             std::vector<uint32_t> v(g->n);
-            e.copy_full(q, v.data());
+            e.copy_full(l, v.data());
             std::lock_guard<std::mutex> lk(mu);
-            full[e.tag_of[q]] = std::move(v);
+            full[e.tag_of[l]] = std::move(v);
           }
         }
       }
+      if (!progressed) std::this_thread::yield();
       if (!done.empty()) {
         std::lock_guard<std::mutex> lk(mu);
         const double t = now_ms();
