@@ -870,8 +870,17 @@ void MultiEngine::launch_top_down(LevelCtr *c) {
         else hipLaunchKernelGGL((k_ms_bfs<4, 8>), dim3(grid), dim3(MBS), 0, stream, a);
       }
     } else {
-      // 20 KB of LDS per block: fill every CU with 7 resident blocks
-      hipLaunchKernelGGL(k_ms_sssp, dim3(7 * g.cus), dim3(MBS), 0, stream, a);
+      // One block per chunk, not a persistent grid: an SSSP level runs for
+      // milliseconds, and short blocks let the work-groups of other streams
+      // (a BFS cohort's levels) take CUs as they free up.  The chunk count
+      // comes from the compaction just launched (one readback).
+      HIPC(hipMemcpyAsync(&h_ctr[level & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost,
+                          stream));
+      HIPC(hipStreamSynchronize(stream));
+      const LevelCtr &h = h_ctr[level & 1];
+      const uint64_t chunks = (h.qw[0] + SSSP_C - 1) / SSSP_C + (h.qw[1] + SSSP_C - 1) / SSSP_C;
+      const uint64_t sg = std::max<uint64_t>(1, std::min<uint64_t>(chunks, 1u << 30));
+      hipLaunchKernelGGL(k_ms_sssp, dim3((uint32_t)sg), dim3(MBS), 0, stream, a);
     }
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(&h_ctr[level & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, stream));
