@@ -162,6 +162,10 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       with neighbour sampling (the labels are the
  *                       component minima, as label propagation gives);
  *                       0: label propagation, as in the reference.
+ *   THV_OPT_CC_UF_STEPS step guard of a union-find link or find (default
+ *                       2^20, >= 1).  A link that needs more steps raises
+ *                       a flag and CC reruns by label propagation; the
+ *                       labels are the same either way.
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -174,6 +178,7 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_HINT_BYTES 9
 #define THV_OPT_SSSP_DELTA 10
 #define THV_OPT_CC_UNION 11
+#define THV_OPT_CC_UF_STEPS 12
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

@@ -311,6 +311,31 @@ def test_cc_union_find(scale, ef, frac):
     assert len(np.unique(ref)) > 1
 
 
+def test_cc_union_find_step_guard_fallback():
+    """A union-find link that needs more steps than the guard allows raises
+    the device flag; CC then reruns by label propagation and the labels
+    are still the component minima.  The guard is lowered from the host,
+    so every kernel still ends normally."""
+    _say("case: cc union-find step guard")
+    g = T.Graph.rmat(14, 16, seed=9, symmetric=True)
+    og = O.rmat(14, 16, seed=9)
+    g.store_config(None, 1 << 12)
+    ref = O.cc(og)
+    lab, st = g.cc()
+    assert np.array_equal(lab, ref)
+    assert st.iterations <= 2            # union-find: two passes
+    g.set_option("cc_uf_steps", 1)
+    assert g.get_option("cc_uf_steps") == 1.0
+    lab1, st1 = g.cc()
+    assert np.array_equal(lab1, ref)
+    assert st1.iterations > 2            # the label-propagation rerun
+    with pytest.raises(T.ThvError):
+        g.set_option("cc_uf_steps", 0)
+    g.set_option("cc_uf_steps", 1 << 20)
+    lab2, st2 = g.cc()
+    assert np.array_equal(lab2, ref) and st2.iterations <= 2
+
+
 def test_pr_push_pull_rmat():
     _say("case: rmat14 directed pagerank")
     og = O.rmat(14, 16, seed=2, symmetric=False)

@@ -121,6 +121,10 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
         g->do_link_gbps = value;
         break;
       case THV_OPT_CC_UNION: g->cc_union = value != 0.0; break;
+      case THV_OPT_CC_UF_STEPS:
+        NEED(value >= 1.0 && value <= 4294967295.0);
+        g->uf_max_steps = (uint32_t)value;
+        break;
       case THV_OPT_SSSP_DELTA:
         NEED(value >= 0.0 && value <= 4294967295.0);
         g->sssp_delta = (uint32_t)value;
@@ -150,6 +154,7 @@ int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
       case THV_OPT_HINT_BYTES: *value = (double)g->store.d_hint.bytes; break;
       case THV_OPT_SSSP_DELTA: *value = g->sssp_delta; break;
       case THV_OPT_CC_UNION: *value = g->cc_union ? 1.0 : 0.0; break;
+      case THV_OPT_CC_UF_STEPS: *value = g->uf_max_steps; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });

@@ -31,7 +31,7 @@ constexpr uint32_t CC_SAMPLES = 1024;
 template <int REC>
 __global__ void __launch_bounds__(UBS)
 k_cc_first(const uint64_t *eptr, const uint32_t *deg, const uint32_t *rbit, uint64_t n,
-           uint32_t *comp, uint32_t *err, unsigned long long *cnt) {
+           uint32_t *comp, uint32_t *err, uint32_t steps, unsigned long long *cnt) {
   __shared__ unsigned long long sh[UBS / 64];
   unsigned long long rd[2] = {0, 0};  // records read: host, HBM
   for (uint64_t v = blockIdx.x * (uint64_t)UBS + threadIdx.x; v < n;
@@ -41,8 +41,8 @@ k_cc_first(const uint64_t *eptr, const uint32_t *deg, const uint32_t *rbit, uint
     const unsigned long long b = eptr[v];
     const uint32_t w0 = *reinterpret_cast<const uint32_t *>(b);
     const uint32_t w1 = d > 1 ? *reinterpret_cast<const uint32_t *>(b + REC) : w0;
-    uf_link((uint32_t)v, w0, comp, err);
-    if (d > 1) uf_link((uint32_t)v, w1, comp, err);
+    uf_link((uint32_t)v, w0, comp, err, steps);
+    if (d > 1) uf_link((uint32_t)v, w1, comp, err, steps);
     rd[(rbit[v >> 5] >> (v & 31)) & 1u] += d > 1 ? 2 : 1;
   }
   const unsigned long long h = block_sum<UBS>(rd[0], sh);
@@ -53,10 +53,11 @@ k_cc_first(const uint64_t *eptr, const uint32_t *deg, const uint32_t *rbit, uint
   }
 }
 
-__global__ void __launch_bounds__(UBS) k_cc_compress(uint32_t *comp, uint64_t n, uint32_t *err) {
+__global__ void __launch_bounds__(UBS) k_cc_compress(uint32_t *comp, uint64_t n, uint32_t *err,
+                                                     uint32_t steps) {
   for (uint64_t v = blockIdx.x * (uint64_t)UBS + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * UBS)
-    comp[v] = uf_find(comp, (uint32_t)v, err);
+    comp[v] = uf_find(comp, (uint32_t)v, err, steps);
 }
 
 __global__ void k_cc_sample(const uint32_t *comp, uint64_t n, uint32_t *out) {
@@ -97,15 +98,16 @@ bool cc_union_find(Graph &g, thv_stats *st) {
   HIPC(hipMemsetAsync(aux, 0, 3 * 8, s));  // [0], [1] records read, [2] error flag
   const uint64_t t0 = now_ns();
   const int grid = grid_for(n, 16 * g.cus);
+  const uint32_t steps = g.uf_max_steps;
   const uint64_t *eptr = g.store.d_eptr.as<uint64_t>();
   const uint32_t *deg = g.d_deg.as<uint32_t>(), *rbit = g.store.d_rbit.as<uint32_t>();
   if (g.rec == 8)
     hipLaunchKernelGGL(k_cc_first<8>, dim3(grid), dim3(UBS), 0, s, eptr, deg, rbit, n, comp, err,
-                       aux);
+                       steps, aux);
   else
     hipLaunchKernelGGL(k_cc_first<4>, dim3(grid), dim3(UBS), 0, s, eptr, deg, rbit, n, comp, err,
-                       aux);
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(UBS), 0, s, comp, n, err);
+                       steps, aux);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(UBS), 0, s, comp, n, err, steps);
   // the sample lands in the probe's short-list buffer (unused here; at
   // least one tile = 16 KB, CC_SAMPLES x 4 = 4 KB), then on the host
   static_assert(CC_SAMPLES * 4 <= TILE * 4, "sample buffer");
@@ -132,10 +134,11 @@ bool cc_union_find(Graph &g, thv_stats *st) {
   a.ctr = c;
   a.val = comp;
   a.err = err;
+  a.uf_steps = steps;
   HIPC(hipEventRecord(w.ev[0], s));
   launch_expand(g, A_CC_LINK, a, s);
   HIPC(hipEventRecord(w.ev[1], s));
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(UBS), 0, s, comp, n, err);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(UBS), 0, s, comp, n, err, steps);
   HIPC(hipMemcpyAsync(w.h_aux, aux, 3 * 8, hipMemcpyDeviceToHost, s));
   HIPC(hipMemcpyAsync(&w.h_ctr[0], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));

@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
     } else if (ALGO == A_CC_LINK) {
 #pragma unroll
       for (int j = 0; j < EPT; j++)
-        if (dst[j] != 0xFFFFFFFFu) uf_link(s_u[ent[j]], dst[j], a.val, a.err);
+        if (dst[j] != 0xFFFFFFFFu) uf_link(s_u[ent[j]], dst[j], a.val, a.err, a.uf_steps);
     } else if (ALGO == A_HINT) {
       uint32_t dw[EPT];
 #pragma unroll

@@ -190,6 +190,7 @@ struct Graph {
   // SSSP: bucket width of delta-stepping (0: frontier Bellman-Ford)
   uint32_t sssp_delta = 16;
   bool cc_union = true;       // CC of a symmetric graph by union-find
+  uint32_t uf_max_steps = 1u << 20;  // union-find step guard (then label propagation)
   // PageRank push: the destinations that receive the most records (from a
   // 1-in-64 sample of the edge array), sorted, and their bitmap; blocks
   // accumulate adds to them in LDS.  Built on the first push call.
@@ -229,6 +230,7 @@ struct ExpandArgs {
   const uint32_t *deg;      // hint build: degrees
   unsigned long long *key64;  // hint build: per-vertex best key
   uint32_t *err;            // CC link: step-guard flag
+  uint32_t uf_steps;        // CC link: step guard
   float *pr_src;            // PR push: delta; PR pull: contribution
   float *pr_dst;            // PR push: residual; PR pull: sum
   const uint32_t *hubs;     // PR push: hub ids (nhubs, sorted) and bitmap
