@@ -53,7 +53,12 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   __shared__ uint32_t s_mask[NW];
   __shared__ uint32_t s_pre[NW];
   __shared__ unsigned long long s_base[C + 1];
-  __shared__ uint32_t s_u[C + 1];  // CC/SSSP: source value; PR pull: vertex id
+  // CC/SSSP: source value; PR pull, hint build, CC link: vertex id (PR push
+  // has no use for it).  PR push runs 2 blocks per CU, bound by LDS: a table
+  // of 1,024 hubs gave 3 blocks per CU but was slower (83 ms against 78 per
+  // expansion at R-MAT 26), the hub atomics weigh more than occupancy.
+  constexpr uint32_t US = ALGO == A_PR_PUSH ? 1 : C + 1;
+  __shared__ uint32_t s_u[US];
   __shared__ float s_f[C + 1];     // PR push: delta; PR pull: partial sum
   // PR push: per-block sums for the hub destinations (an open-addressing
   // table of the hub ids, built at kernel start, flushed at the end)
