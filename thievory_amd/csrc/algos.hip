@@ -276,7 +276,6 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   LevelCtr *c_long = d_ctr + 2, *c_short = d_ctr + 3;
   const uint64_t rec = (uint64_t)g.rec;
   const uint32_t D = g.do_pull_degree;
-  const uint32_t *vis = w.vis.as<uint32_t>();
   int cur = 0;
   bool bu = false;
   const bool trace = getenv("THV_TRACE") != nullptr;
@@ -440,14 +439,13 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       if (!probed) probe(hint, next);
       HIPC(hipEventRecord(w.ev[0], s));
       // the probe scanned the short lists already (and, with hints, settled
-      // every degree-1 vertex by its hint); the long set is the one it
-      // counted, so the same degree bounds select it here
-      launch_compact(g, w.qs, vis, c_long, s, CMP_INVERT | CMP_NO_CHUNKS | CMP_WRITE_ONLY, nullptr,
-                     0, std::max<uint32_t>(D + 1, hint ? 2u : 1u), 0xFFFFFFFFu, 1);
+      // every degree-1 vertex by its hint) and wrote the long ones, per wave
+      // region, for the early-exit kernel
       BottomUpArgs b;
-      b.qv[0] = w.qs.qv[0].as<uint32_t>();
-      b.qv[1] = w.qs.qv[1].as<uint32_t>();
-      b.ctr = c_long;
+      b.llist = w.qs.llist.as<uint32_t>();
+      b.lcnt = w.qs.lcnt.as<uint32_t>();
+      b.nregions = (uint32_t)(div_up(g.n, TILE) * (TILE / PROBE_REGION));
+      b.rbit = g.store.d_rbit.as<uint32_t>();
       b.eptr = g.store.d_eptr.as<uint64_t>();
       b.deg = g.d_deg.as<uint32_t>();
       b.front = w.front[cur].as<uint32_t>();

@@ -7,12 +7,12 @@
 // top-down's (the reference only has the top-down rule, bfs_kernels.cu:
 // 29-44); what changes is how many edge records cross the host link.
 //
-// One group of 16 lanes per unvisited vertex (queue from the compaction
-// of the inverted visited bitmap): the group reads 16 consecutive records
-// (64 bytes of 4-byte records) per step and tests the frontier bitmap; a
-// ballot ends the scan at the first hit.  Host and HBM queue entries are
-// interleaved like the expansion chunks.  Records read are counted per
-// residency for the byte accounting.
+// One group of 16 lanes per unvisited vertex with a long list (the probe
+// wrote them per wave region, in vertex order): the group reads 16
+// consecutive records (64 bytes of 4-byte records) per step and tests the
+// frontier bitmap; a ballot ends the scan at the first hit.  A wave walks
+// one region at a time.  Records read are counted per residency for the
+// byte accounting.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -28,16 +28,14 @@ constexpr int G = 16;  // lanes per vertex
 template <int REC>
 __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
   const uint32_t lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
-  const unsigned long long group = (blockIdx.x * (unsigned long long)BBS + threadIdx.x) / G;
-  const unsigned long long ngroups = (unsigned long long)gridDim.x * BBS / G;
-  const unsigned long long nh = a.ctr->qn[Q_HOST], nd = a.ctr->qn[Q_DEV];
-  const unsigned long long total = nh + nd;
+  const uint32_t nwaves = gridDim.x * (BBS / 64);
   unsigned long long rd[2] = {0, 0};
-  for (unsigned long long i = group; i < total; i += ngroups) {
-    const unsigned long long d0 = i * nd / total, d1 = (i + 1) * nd / total;
-    const int q = d1 > d0 ? Q_DEV : Q_HOST;
-    const unsigned long long idx = q == Q_DEV ? d0 : i - d0;
-    const uint32_t v = a.qv[q][idx];
+  for (uint32_t rg = blockIdx.x * (BBS / 64) + (threadIdx.x >> 6); rg < a.nregions;
+       rg += nwaves) {
+   const uint32_t cnt = a.lcnt[rg];
+   for (uint32_t i = gw; i < cnt; i += 64 / G) {
+    const uint32_t v = a.llist[(uint64_t)rg * PROBE_REGION + i];
+    const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
     const uint32_t d = a.deg[v];
     const unsigned long long base = a.eptr[v];
     bool found = false;
@@ -62,6 +60,7 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
         atomicOr(&a.vis[v >> 5], 1u << (v & 31));
       }
     }
+   }
   }
   const unsigned long long h = wave_sum(rd[Q_HOST]), dv = wave_sum(rd[Q_DEV]);
   if (lane == 0) {

@@ -34,6 +34,7 @@ constexpr int CBS = 256;
 constexpr uint32_t SEG = 64;  // tiles per segment of the two-level scan
 constexpr uint32_t SUB = TILE / (CBS / 64);  // vertices per wave
 constexpr uint32_t ROUNDS = SUB / 64;
+static_assert(SUB == PROBE_REGION, "probe regions are the waves' sub-tiles");
 
 struct CompactArgs {
   const uint32_t *front, *rbit, *deg, *vpart;
@@ -376,6 +377,7 @@ struct ProbeArgs {
   unsigned long long *cnt;  // [0] hint hits, [1] unvisited, [2] with a hint, [3] words written
   uint32_t *slist;          // short lists: wave (tile, w) writes from slot tile*TILE + w*SUB
   uint32_t *scnt;           //   and their count at [tile * 4 + w]
+  uint32_t *llist, *lcnt;   // long lists (degree > D), same layout
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
@@ -387,7 +389,8 @@ constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
 //      and is written into the wave's own visited / next words by the lane
 //      that loaded them;
 //   3. degrees of the rest.  Lists of 1..D records go to the wave's slots
-//      of the short-list array, in vertex order, for k_bu_short.
+//      of the short-list array, in vertex order, for k_bu_short; longer
+//      ones to the long-list array, for the early-exit kernel.
 //   Counts: tile set 0 holds the short lists (entries, records), tile set
 //   1 the long unvisited lists (degree > D) left for the early-exit kernel,
 //   both split by residency.
@@ -409,7 +412,10 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
       if (a.n - first < 32) uw &= (1u << (a.n - first)) - 1u;
     }
     if (!__syncthreads_or(uw != 0)) {
-      if (lane == 0) a.scnt[tile * (CBS / 64) + wv] = 0;
+      if (lane == 0) {
+        a.scnt[tile * (CBS / 64) + wv] = 0;
+        a.lcnt[tile * (CBS / 64) + wv] = 0;
+      }
       if (threadIdx.x < 8) {
         const uint32_t set = threadIdx.x >> 2, i = threadIdx.x & 3;
         if (i < 2) a.tcnt[set][2 * tile + i] = 0;
@@ -460,20 +466,27 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     const uint32_t dlo = a.hint ? 2u : 1u;
     uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
     unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
-    uint32_t ns = 0;
+    uint32_t ns = 0, nl = 0;
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
       const uint32_t q = (dev >> r) & 1u;
       const bool sh = d[r] >= dlo && d[r] <= a.D;
+      const bool lg = d[r] >= dlo && d[r] > a.D;
       const unsigned long long b = __ballot(sh);  // every lane takes part
+      const unsigned long long bl = __ballot(lg);
       if (sh) a.slist[base + ns + __popcll(b & lt)] = (uint32_t)(base + r * 64 + lane);
+      if (lg) a.llist[base + nl + __popcll(bl & lt)] = (uint32_t)(base + r * 64 + lane);
       ns += __popcll(b);
+      nl += __popcll(bl);
       if (d[r] < dlo) continue;
       const uint32_t k = (sh ? 0 : 2) + q;
       xc[k] += 1;
       xe[k] += d[r];
     }
-    if (lane == 0) a.scnt[tile * (CBS / 64) + wv] = ns;
+    if (lane == 0) {
+      a.scnt[tile * (CBS / 64) + wv] = ns;
+      a.lcnt[tile * (CBS / 64) + wv] = nl;
+    }
     hits += __popc(m);  // hint hits
     if (owner && m) {
       a.vis[w0 + lane] = vw | m;
@@ -669,6 +682,8 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   if (D > SHORT_MAX) fail(THV_EINVAL, "bottom-up full-scan degree above 16");
   p.slist = w.slist.as<uint32_t>();
   p.scnt = w.scnt.as<uint32_t>();
+  p.llist = w.llist.as<uint32_t>();
+  p.lcnt = w.lcnt.as<uint32_t>();
   const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
   if (t0) HIPC(hipEventRecord(t0, s));
   hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);

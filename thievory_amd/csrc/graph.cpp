@@ -394,6 +394,8 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
   if (slist.bytes < tiles * TILE * 4) {
     slist.alloc(tiles * TILE * 4);
     scnt.alloc(tiles * 4 * 4);
+    llist.alloc(tiles * TILE * 4);
+    lcnt.alloc(tiles * 4 * 4);
   }
   for (int k = 0; k < 2; k++) {
     if (tcnt[k].bytes < tiles * 2 * 4) {

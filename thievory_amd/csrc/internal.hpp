@@ -117,6 +117,7 @@ struct Graph;
 // consumed by an expansion kernel.  Engines that run concurrently on
 // different streams own separate QueueSets.
 constexpr uint32_t TILE = 4096;  // vertices per compaction tile
+constexpr uint32_t PROBE_REGION = 1024;  // vertices per wave region of the bottom-up probe
 
 struct QueueSet {
   uint64_t n = 0, m = 0;
@@ -132,6 +133,7 @@ struct QueueSet {
                               // between counts) and exclusive bases
   DevMem ptouch;              // per partition: stamp of the last touching call
   DevMem slist, scnt;         // bottom-up probe: short lists per wave region
+  DevMem llist, lcnt;         //   and long lists (for the early-exit kernel)
   void ensure(const Graph &g, uint32_t C);
 };
 
@@ -273,8 +275,9 @@ void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vi
                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 
 struct BottomUpArgs {
-  const uint32_t *qv[2];
-  const LevelCtr *ctr;        // queue of unvisited vertices (qn)
+  const uint32_t *llist, *lcnt;  // the probe's long lists, per wave region
+  uint32_t nregions;
+  const uint32_t *rbit;       // residency, for the records-read counts
   const uint64_t *eptr;
   const uint32_t *deg;
   const uint32_t *front;      // current frontier
