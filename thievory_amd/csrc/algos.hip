@@ -297,9 +297,15 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   uint32_t launches = 0, probes = 0;
   uint64_t probe_bytes = 0;  // the parts that do not come from d_pc
   HIPC(hipMemsetAsync(d_pc, 0, 4 * 8, s));
-  auto probe = [&](const uint32_t *hint, uint32_t *next) {
+  // a bottom-up level writes the values of the vertices it places and sums
+  // the next frontier (c_next), so the next level needs no count pass
+  LevelCtr *c_next = d_ctr + 4;
+  bool fused_next = false;
+  auto probe = [&](const uint32_t *hint, uint32_t *next, uint32_t level_value) {
+    HIPC(hipMemsetAsync(c_next, 0, sizeof(LevelCtr), s));
     launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint, D,
-                    c_short, c_long, d_pc, s, w.ev[2], w.ev[3]);
+                    c_short, c_long, d_pc, s, w.ev[2], w.ev[3], c_next, w.val.as<uint32_t>(),
+                    level_value);
     probes++;
     // visited and residency words, tile tables of both sets
     probe_bytes += 2 * 4 * words + 2 * 24 * ((g.n + TILE - 1) / TILE);
@@ -378,9 +384,16 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
-    launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_COUNT_ONLY,
-                   w.val.as<uint32_t>(), L + 1);
-    HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    // fused: the last level ran bottom-up, wrote this frontier's values and
+    // summed it; otherwise count it (and write its values) now
+    const bool fused = fused_next;
+    if (fused) {
+      HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c_next, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    } else {
+      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_COUNT_ONLY,
+                     w.val.as<uint32_t>(), L + 1);
+      HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    }
     HIPC(hipStreamSynchronize(s));
     finish_level(pend);
     pend.valid = false;
@@ -408,7 +421,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       const double probe_cost = (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED;
       bu = false;
       if (td_cost > probe_cost) {
-        probe(hint, next);
+        probe(hint, next, L + 2);
         HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
         if (trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
@@ -428,15 +441,17 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     prev_nf = nf;
     a.next = next;
     if (!bu) {
-      // the probe replaced the tile counts: a full compaction of the frontier
-      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, probed ? 0 : CMP_WRITE_ONLY);
+      // without a count of this frontier in the tile tables (the probe
+      // replaced them, or the count was fused away): a full compaction
+      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s,
+                     probed || fused ? 0 : CMP_WRITE_ONLY);
       a.ctr = c;
       a.level_value = L + 2;
       HIPC(hipEventRecord(w.ev[0], s));
       launch_expand(g, A_BFS, a, s);
       HIPC(hipEventRecord(w.ev[1], s));
     } else {
-      if (!probed) probe(hint, next);
+      if (!probed) probe(hint, next, L + 2);
       HIPC(hipEventRecord(w.ev[0], s));
       // the probe scanned the short lists already (and, with hints, settled
       // every degree-1 vertex by its hint) and wrote the long ones, per wave
@@ -454,6 +469,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       b.val = w.val.as<uint32_t>();
       b.rd_ctr = c_long;
       b.level_value = L + 2;
+      b.nctr = c_next;
       launch_bottom_up(g, b, s);
       HIPC(hipEventRecord(w.ev[1], s));
       HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
@@ -468,6 +484,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     pend.h = h;
     pend.td_cost = td_cost;
     pend.bu_cost = bu_cost;
+    fused_next = bu;
     cur ^= 1;
   }
   if (st) {
@@ -482,11 +499,11 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       st->probe_ms = probe_ms;
       st->probe_launches = probes;
       // per unvisited vertex: its hint word (when hints are on) and, when
-      // it has one, the frontier word it names; its degree unless a hint
-      // placed it; 12 bytes per bitmap word updated (visited written,
-      // next read and written)
-      st->probe_hbm_bytes = probe_bytes + (hints ? 4 * pc[1] + 4 * pc[2] : 0) +
-                            4 * (pc[1] - pc[0]) + 12 * pc[3];
+      // it has one, the frontier word it names, and its degree; the value
+      // of each vertex a hint placed; 12 bytes per bitmap word updated
+      // (visited written, next read and written)
+      st->probe_hbm_bytes = probe_bytes + (hints ? 4 * pc[1] + 4 * pc[2] : 0) + 4 * pc[1] +
+                            4 * pc[0] + 12 * pc[3];
       st->hbm_bytes += st->probe_hbm_bytes;
     }
   }

@@ -27,9 +27,10 @@ constexpr int G = 16;  // lanes per vertex
 
 template <int REC>
 __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
+  __shared__ unsigned long long sh[BBS / 64];
   const uint32_t lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
   const uint32_t nwaves = gridDim.x * (BBS / 64);
-  unsigned long long rd[2] = {0, 0};
+  unsigned long long rd[2] = {0, 0}, fn[2] = {0, 0}, fe[2] = {0, 0};
   for (uint32_t rg = blockIdx.x * (BBS / 64) + (threadIdx.x >> 6); rg < a.nregions;
        rg += nwaves) {
    const uint32_t cnt = a.lcnt[rg];
@@ -55,17 +56,34 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
     }
     if (gl == 0) {
       rd[q] += found ? min(d, k + G) : d;
-      if (found) {  // level written by the next compaction, like top-down
+      if (found) {
         atomicOr(&a.next[v >> 5], 1u << (v & 31));
         atomicOr(&a.vis[v >> 5], 1u << (v & 31));
+        if (a.nctr) {  // else the next level's count writes the level
+          a.val[v] = a.level_value;
+          fn[q] += 1;
+          fe[q] += d;
+        }
       }
     }
    }
   }
-  const unsigned long long h = wave_sum(rd[Q_HOST]), dv = wave_sum(rd[Q_DEV]);
-  if (lane == 0) {
+  // one atomic per counter and block; every thread takes part in the sums
+  const unsigned long long h = block_sum<BBS>(rd[Q_HOST], sh);
+  const unsigned long long dv = block_sum<BBS>(rd[Q_DEV], sh);
+  if (threadIdx.x == 0) {
     if (h) atomicAdd(&a.rd_ctr->rd[Q_HOST], h);
     if (dv) atomicAdd(&a.rd_ctr->rd[Q_DEV], dv);
+  }
+  if (a.nctr) {
+    const unsigned long long x0 = block_sum<BBS>(fn[0], sh), x1 = block_sum<BBS>(fn[1], sh);
+    const unsigned long long y0 = block_sum<BBS>(fe[0], sh), y1 = block_sum<BBS>(fe[1], sh);
+    if (threadIdx.x == 0) {
+      if (x0) atomicAdd(&a.nctr->qn[0], x0);
+      if (x1) atomicAdd(&a.nctr->qn[1], x1);
+      if (y0) atomicAdd(&a.nctr->qw[0], y0);
+      if (y1) atomicAdd(&a.nctr->qw[1], y1);
+    }
   }
 }
 

@@ -378,6 +378,13 @@ struct ProbeArgs {
   uint32_t *slist;          // short lists: wave (tile, w) writes from slot tile*TILE + w*SUB
   uint32_t *scnt;           //   and their count at [tile * 4 + w]
   uint32_t *llist, *lcnt;   // long lists (degree > D), same layout
+  // next-frontier bookkeeping (nctr != nullptr): every vertex this level
+  // places gets its value here, and nctr sums the entries and edges of the
+  // next frontier per residency, so a following bottom-up level needs no
+  // count pass
+  uint32_t *val;
+  uint32_t level_value;
+  LevelCtr *nctr;
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
@@ -399,6 +406,8 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = lanemask_lt();
   unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0;
+  unsigned long long fe[2] = {0, 0};  // next frontier: edges (host, HBM)
+  uint32_t fn[2] = {0, 0};            //   and entries
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB, w0 = base >> 5;
     const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
@@ -434,6 +443,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     }
     nun += __popc(un);
     uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
+    uint32_t f = 0;  // bit r: round r's vertex is a hint hit
     if (a.hint) {
       uint32_t h[ROUNDS];
 #pragma unroll
@@ -446,7 +456,6 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
       uint32_t fw[ROUNDS];
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++) fw[r] = a.front[h[r] != NO_HINT ? h[r] >> 5 : 0];
-      uint32_t f = 0;
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++)
         if (h[r] != NO_HINT && ((fw[r] >> (h[r] & 31)) & 1u)) f |= 1u << r;
@@ -456,11 +465,24 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
         if (lane == 2 * r) m = (uint32_t)b;
         if (lane == 2 * r + 1) m = (uint32_t)(b >> 32);
       }
-      un &= ~f;
+      // with the bookkeeping on, the hits' degrees load with the others'
+      if (!a.nctr) un &= ~f;
     }
     uint32_t d[ROUNDS];
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
+    if (a.nctr && f) {
+#pragma unroll
+      for (uint32_t r = 0; r < ROUNDS; r++) {
+        if (!((f >> r) & 1u)) continue;
+        const uint32_t q = (dev >> r) & 1u;
+        a.val[base + r * 64 + lane] = a.level_value;
+        fn[q] += 1;
+        fe[q] += d[r];
+        d[r] = 0;  // placed: no list to scan
+      }
+      un &= ~f;
+    }
     // counts, and the wave's short lists (in vertex order).  With hints, a
     // degree-1 vertex is settled already: its hint is its one neighbour.
     const uint32_t dlo = a.hint ? 2u : 1u;
@@ -515,17 +537,25 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     __syncthreads();
   }
   // one atomic per counter and block (the grid is a few blocks per CU)
-  const unsigned long long c4[4] = {wave_sum(hits), wave_sum(nun), wave_sum(nhint),
-                                    wave_sum(nwords)};
+  const unsigned long long c8[8] = {wave_sum(hits),  wave_sum(nun),
+                                    wave_sum(nhint), wave_sum(nwords),
+                                    wave_sum((unsigned long long)fn[0]),
+                                    wave_sum((unsigned long long)fn[1]),
+                                    wave_sum(fe[0]), wave_sum(fe[1])};
   if (lane == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) s[wv][i] = c4[i];
+    for (int i = 0; i < 8; i++) s[wv][i] = c8[i];
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < 8) {
     unsigned long long t = 0;
     for (int w = 0; w < CBS / 64; w++) t += s[w][threadIdx.x];
-    if (t) atomicAdd(&a.cnt[threadIdx.x], t);
+    if (t) {
+      const uint32_t i = threadIdx.x;
+      if (i < 4) atomicAdd(&a.cnt[i], t);
+      else if (i < 6) atomicAdd(&a.nctr->qn[i - 4], t);
+      else atomicAdd(&a.nctr->qw[i - 6], t);
+    }
   }
 }
 
@@ -534,6 +564,10 @@ struct ShortArgs {
   const uint64_t *eptr;
   uint32_t *vis, *next;
   uint32_t nregions;
+  const uint32_t *rbit;     // next-frontier bookkeeping, as in ProbeArgs
+  uint32_t *val;
+  uint32_t level_value;
+  LevelCtr *nctr;
 };
 
 // The short lists the probe collected: one wave per region (a wave's slots
@@ -542,7 +576,9 @@ struct ShortArgs {
 // vertex places its vertex.
 template <int REC>
 __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
+  __shared__ unsigned long long sh[CBS / 64];
   const uint32_t lane = threadIdx.x & 63;
+  unsigned long long fe[2] = {0, 0}, fn[2] = {0, 0};
   const uint32_t nwaves = gridDim.x * (CBS / 64);
   for (uint32_t rg = blockIdx.x * (CBS / 64) + (threadIdx.x >> 6); rg < a.nregions;
        rg += nwaves) {
@@ -576,7 +612,23 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
       if (hit) {
         atomicOr(&a.vis[v >> 5], 1u << (v & 31));
         atomicOr(&a.next[v >> 5], 1u << (v & 31));
+        if (a.nctr) {
+          const uint32_t q = (a.rbit[v >> 5] >> (v & 31)) & 1u;
+          a.val[v] = a.level_value;
+          fn[q] += 1;
+          fe[q] += dl;
+        }
       }
+    }
+  }
+  if (a.nctr) {  // every thread takes part in the block sums
+    const unsigned long long x0 = block_sum<CBS>(fn[0], sh), x1 = block_sum<CBS>(fn[1], sh);
+    const unsigned long long y0 = block_sum<CBS>(fe[0], sh), y1 = block_sum<CBS>(fe[1], sh);
+    if (threadIdx.x == 0) {
+      if (x0) atomicAdd(&a.nctr->qn[0], x0);
+      if (x1) atomicAdd(&a.nctr->qn[1], x1);
+      if (y0) atomicAdd(&a.nctr->qw[0], y0);
+      if (y1) atomicAdd(&a.nctr->qw[1], y1);
     }
   }
 }
@@ -659,7 +711,7 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
 void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis,
                      uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s, hipEvent_t t0,
-                     hipEvent_t t1) {
+                     hipEvent_t t1, LevelCtr *nctr, uint32_t *val, uint32_t level_value) {
   ProbeArgs p;
   memset(&p, 0, sizeof p);
   p.front = front;
@@ -684,6 +736,10 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   p.scnt = w.scnt.as<uint32_t>();
   p.llist = w.llist.as<uint32_t>();
   p.lcnt = w.lcnt.as<uint32_t>();
+  p.nctr = nctr;
+  p.val = val;
+  p.level_value = level_value;
+  if (nctr && !val) fail(THV_ESTATE, "next-frontier bookkeeping needs the value array");
   const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
   if (t0) HIPC(hipEventRecord(t0, s));
   hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);
@@ -697,6 +753,10 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
     sa.vis = vis;
     sa.next = next;
     sa.nregions = p.ntiles * (CBS / 64);
+    sa.rbit = p.rbit;
+    sa.val = val;
+    sa.level_value = level_value;
+    sa.nctr = nctr;
     if (g.rec == 8) hipLaunchKernelGGL(k_bu_short<8>, dim3(grid), dim3(CBS), 0, s, sa);
     else hipLaunchKernelGGL(k_bu_short<4>, dim3(grid), dim3(CBS), 0, s, sa);
   }

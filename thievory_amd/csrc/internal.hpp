@@ -108,8 +108,8 @@ struct LevelCtr {
 };
 
 // [0], [1]: alternating levels; [2]: bottom-up early-exit queue; [3]:
-// bottom-up full-scan queue
-enum { CTR_SLOTS = 4 };
+// bottom-up full-scan queue; [4]: next frontier placed by a bottom-up level
+enum { CTR_SLOTS = 5 };
 
 struct Graph;
 
@@ -272,7 +272,9 @@ void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr
 void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vis,
                      uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s,
-                     hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+                     hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
+                     LevelCtr *nctr = nullptr, uint32_t *val = nullptr,
+                     uint32_t level_value = 0);
 
 struct BottomUpArgs {
   const uint32_t *llist, *lcnt;  // the probe's long lists, per wave region
@@ -284,6 +286,7 @@ struct BottomUpArgs {
   uint32_t *vis, *next, *val;
   LevelCtr *rd_ctr;           // receives rd[2]
   uint32_t level_value;
+  LevelCtr *nctr;             // next-frontier bookkeeping (see launch_bu_probe)
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
 void build_parent_hints(Graph &g);
