@@ -76,6 +76,37 @@ def test_batch_bfs_bottom_up_levels(weights, frac):
     assert bu["stats"].edges_traversed < td["stats"].edges_traversed
 
 
+@pytest.mark.parametrize("n", [5, 63, 1000])
+def test_batch_bottom_up_small_graphs(n):
+    """Bottom-up batch levels on small symmetric graphs whose vertex count is
+    not a multiple of a wave (partial last group, one-word bitmaps): every
+    lane equals the oracle."""
+    _say("case: batch bottom-up small", n)
+    rng = np.random.default_rng(n)
+    k = max(1, 3 * n)
+    u = rng.integers(0, n, k)
+    v = rng.integers(0, n, k)
+    keep = u != v
+    pairs = np.unique(np.stack([np.concatenate([u[keep], v[keep]]),
+                                np.concatenate([v[keep], u[keep]])], 1), axis=0)
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, pairs[:, 0] + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    og = O.CSR(off, pairs[:, 1].astype(np.uint32))
+    g = T.Graph.from_arrays(og.off, og.adj)
+    g.set_option("symmetric", 1)
+    srcs = (np.arange(64) * 7 % n).astype(np.uint32)
+    out = g.batch("bfs", srcs, full=True)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(out["full"][i], O.bfs(og, int(s))), (n, i, s)
+    # single-source direction-optimising BFS on the same graph, all edges in
+    # HBM and all zero-copy
+    for frac in (1.0, 0.0):
+        g.store_config(int(frac * og.m * 4), 16)
+        for s in srcs[:8]:
+            assert np.array_equal(g.bfs(int(s))[0], O.bfs(og, int(s))), (n, frac, s)
+
+
 def test_serving_bottom_up_staggered():
     """A server on a symmetric graph admits queries between levels, so its
     lanes run at different depths; levels may still run bottom-up, and each
