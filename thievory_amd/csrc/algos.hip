@@ -458,7 +458,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       // region, for the early-exit kernel
       BottomUpArgs b;
       b.llist = w.qs.llist.as<uint32_t>();
-      b.lcnt = w.qs.lcnt.as<uint32_t>();
+      b.loff = w.qs.loff.as<uint32_t>();
       b.nregions = (uint32_t)(div_up(g.n, TILE) * (TILE / PROBE_REGION));
       b.rbit = g.store.d_rbit.as<uint32_t>();
       b.eptr = g.store.d_eptr.as<uint64_t>();

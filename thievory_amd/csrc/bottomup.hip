@@ -8,11 +8,13 @@
 // 29-44); what changes is how many edge records cross the host link.
 //
 // One group of 16 lanes per unvisited vertex with a long list (the probe
-// wrote them per wave region, in vertex order): the group reads 16
-// consecutive records (64 bytes of 4-byte records) per step and tests the
-// frontier bitmap; a ballot ends the scan at the first hit.  A wave walks
-// one region at a time.  Records read are counted per residency for the
-// byte accounting.
+// wrote them per wave region, in vertex order, and k_lbase the regions'
+// bases): the group reads 16 consecutive records (64 bytes of 4-byte
+// records) per step and tests the frontier bitmap; a ballot ends the scan
+// at the first hit.  Every wave takes an equal run of the lists in region
+// order, wherever they cluster (on R-MAT graphs the long lists crowd into
+// the regions of the low-numbered vertices).  Records read are counted per
+// residency for the byte accounting.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -29,44 +31,70 @@ template <int REC>
 __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
   __shared__ unsigned long long sh[BBS / 64];
   const uint32_t lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
-  const uint32_t nwaves = gridDim.x * (BBS / 64);
+  const uint64_t nwaves = (uint64_t)gridDim.x * (BBS / 64);
+  const uint64_t wave = (uint64_t)blockIdx.x * (BBS / 64) + (threadIdx.x >> 6);
   unsigned long long rd[2] = {0, 0}, fn[2] = {0, 0}, fe[2] = {0, 0};
-  for (uint32_t rg = blockIdx.x * (BBS / 64) + (threadIdx.x >> 6); rg < a.nregions;
-       rg += nwaves) {
-   const uint32_t cnt = a.lcnt[rg];
-   for (uint32_t i = gw; i < cnt; i += 64 / G) {
-    const uint32_t v = a.llist[(uint64_t)rg * PROBE_REGION + i];
-    const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
-    const uint32_t d = a.deg[v];
-    const unsigned long long base = a.eptr[v];
-    bool found = false;
-    uint32_t k = 0;
-    for (; k < d; k += G) {
-      const uint32_t e = k + gl;
-      bool hit = false;
-      if (e < d) {
-        const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (unsigned long long)e * REC);
-        hit = (a.front[w >> 5] >> (w & 31)) & 1u;
-      }
-      const unsigned long long b = __ballot(hit);
-      if ((b >> (gw * G)) & 0xFFFFull) {
-        found = true;
-        break;
-      }
+  // this wave's run [s, e) of the T lists
+  const uint64_t T = a.loff[a.nregions];
+  const uint64_t per = (T + nwaves - 1) / nwaves;
+  const uint64_t s = wave * per, e = s + per < T ? s + per : T;
+  if (s < e) {
+    // the region holding list s: the last region whose base is <= s
+    // (64 candidates per round; bases are non-decreasing and loff[nregions]
+    // = T > s)
+    uint32_t lo = 0, hi = a.nregions;
+    while (hi - lo > 1) {
+      const uint32_t step = (hi - lo + 63) / 64;
+      const uint64_t p = (uint64_t)lo + (uint64_t)lane * step;
+      const bool le = p < hi && a.loff[p] <= s;
+      const uint32_t c = (uint32_t)__popcll(__ballot(le));  // lane 0: p = lo
+      lo += (c - 1) * step;
+      if ((uint64_t)lo + step < hi) hi = lo + step;
     }
-    if (gl == 0) {
-      rd[q] += found ? min(d, k + G) : d;
-      if (found) {
-        atomicOr(&a.next[v >> 5], 1u << (v & 31));
-        atomicOr(&a.vis[v >> 5], 1u << (v & 31));
-        if (a.nctr) {  // else the next level's count writes the level
-          a.val[v] = a.level_value;
-          fn[q] += 1;
-          fe[q] += d;
+    uint32_t rg = lo;  // per group from here on: the region of its list
+    for (uint64_t i0 = s; i0 < e; i0 += 64 / G) {
+      const uint64_t i = i0 + gw;
+      if (i >= e) break;
+      // move on to the region holding list i, 16 bases per step
+      for (;;) {
+        const uint64_t r = (uint64_t)rg + 1 + gl;
+        const bool le = r <= a.nregions && a.loff[r] <= i;
+        const uint32_t c = (uint32_t)__popcll((__ballot(le) >> (gw * G)) & 0xFFFFull);
+        rg += c;
+        if (c < G) break;
+      }
+      const uint32_t v = a.llist[(uint64_t)rg * PROBE_REGION + (i - a.loff[rg])];
+      const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
+      const uint32_t d = a.deg[v];
+      const unsigned long long base = a.eptr[v];
+      bool found = false;
+      uint32_t k = 0;
+      for (; k < d; k += G) {
+        const uint32_t x = k + gl;
+        bool hit = false;
+        if (x < d) {
+          const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (unsigned long long)x * REC);
+          hit = (a.front[w >> 5] >> (w & 31)) & 1u;
+        }
+        const unsigned long long b = __ballot(hit);
+        if ((b >> (gw * G)) & 0xFFFFull) {
+          found = true;
+          break;
+        }
+      }
+      if (gl == 0) {
+        rd[q] += found ? min(d, k + G) : d;
+        if (found) {
+          atomicOr(&a.next[v >> 5], 1u << (v & 31));
+          atomicOr(&a.vis[v >> 5], 1u << (v & 31));
+          if (a.nctr) {  // else the next level's count writes the level
+            a.val[v] = a.level_value;
+            fn[q] += 1;
+            fe[q] += d;
+          }
         }
       }
     }
-   }
   }
   // one atomic per counter and block; every thread takes part in the sums
   const unsigned long long h = block_sum<BBS>(rd[Q_HOST], sh);

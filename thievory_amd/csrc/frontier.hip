@@ -633,6 +633,28 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
   }
 }
 
+// Bases of the probe's long-list regions: one block per segment of SEG
+// tiles (SEG * 4 regions, one per thread), on top of the segment's base
+// from k_sscan (both residencies).  The last block writes the total after
+// the last region, so the early-exit kernel can split the lists evenly.
+struct LbaseArgs {
+  const uint32_t *lcnt;
+  const unsigned long long *sbase;  // set 1 (long lists) segment bases
+  uint32_t *loff;
+  uint32_t nregions, nseg;
+};
+constexpr int LBS = SEG * (CBS / 64);
+__global__ void __launch_bounds__(LBS) k_lbase(LbaseArgs a) {
+  __shared__ uint32_t sh[LBS / 64 + 1];
+  const uint32_t rg = blockIdx.x * LBS + threadIdx.x;
+  const uint32_t x = rg < a.nregions ? a.lcnt[rg] : 0u;
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan<LBS>(x, sh, &tot);
+  const uint32_t base = (uint32_t)(a.sbase[blockIdx.x * 4] + a.sbase[blockIdx.x * 4 + 1]);
+  if (rg < a.nregions) a.loff[rg] = base + ex;
+  if (blockIdx.x == a.nseg - 1 && threadIdx.x == 0) a.loff[a.nregions] = base + tot;
+}
+
 // Partitions stamped by this call, per residency.
 __global__ void __launch_bounds__(CBS) k_touched(CompactArgs a) {
   __shared__ unsigned long long sh[CBS / 64];
@@ -774,6 +796,13 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
     a.ctr = k == 0 ? cs : cl;
     hipLaunchKernelGGL(k_sscan, dim3(1), dim3(SBS), 0, s, a);
   }
+  LbaseArgs lb;
+  lb.lcnt = p.lcnt;
+  lb.sbase = w.sbase[1].as<unsigned long long>();
+  lb.loff = w.loff.as<uint32_t>();
+  lb.nregions = p.ntiles * (CBS / 64);
+  lb.nseg = (uint32_t)div_up(p.ntiles, SEG);
+  hipLaunchKernelGGL(k_lbase, dim3(lb.nseg), dim3(LBS), 0, s, lb);
   HIPC(hipGetLastError());
 }
 

@@ -396,6 +396,7 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
     scnt.alloc(tiles * 4 * 4);
     llist.alloc(tiles * TILE * 4);
     lcnt.alloc(tiles * 4 * 4);
+    loff.alloc((tiles * 4 + 1) * 4);
   }
   for (int k = 0; k < 2; k++) {
     if (tcnt[k].bytes < tiles * 2 * 4) {

@@ -134,6 +134,7 @@ struct QueueSet {
   DevMem ptouch;              // per partition: stamp of the last touching call
   DevMem slist, scnt;         // bottom-up probe: short lists per wave region
   DevMem llist, lcnt;         //   and long lists (for the early-exit kernel)
+  DevMem loff;                //   with their regions' exclusive bases (+ total)
   void ensure(const Graph &g, uint32_t C);
 };
 
@@ -277,8 +278,8 @@ void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vi
                      uint32_t level_value = 0);
 
 struct BottomUpArgs {
-  const uint32_t *llist, *lcnt;  // the probe's long lists, per wave region
-  uint32_t nregions;
+  const uint32_t *llist, *loff;  // the probe's long lists, per wave region,
+  uint32_t nregions;             //   and the regions' bases (loff[nregions]: total)
   const uint32_t *rbit;       // residency, for the records-read counts
   const uint64_t *eptr;
   const uint32_t *deg;
