@@ -143,13 +143,15 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       bottom-up level as (bytes of those short lists on
  *                       the host / link rate) + (longer unvisited host
  *                       lists x DO_VERTEX_NS).
- *   THV_OPT_PARENT_HINTS  1 (default): on a split store of a symmetric graph
- *                       with direction-optimising BFS, n x 4 bytes of the
- *                       HBM budget hold one parent hint per vertex -- its
- *                       neighbour of largest degree.  A bottom-up level first
- *                       tests the hints; a vertex whose hint is in the
- *                       frontier joins the next level without its list
- *                       being read.  0: no hints, the whole budget holds
+ *   THV_OPT_PARENT_HINTS  1 (default): a symmetric graph with
+ *                       direction-optimising BFS keeps one parent hint per
+ *                       vertex in HBM -- its neighbour of largest degree.
+ *                       A bottom-up level first tests the hints; a vertex
+ *                       whose hint is in the frontier joins the next level
+ *                       without its list being read.  A resident store
+ *                       holds them beside the lists (n x 4 bytes more); a
+ *                       split store with 4-byte records takes them from the
+ *                       budget.  0: no hints, the whole budget holds
  *                       partitions.  Changing it, SYMMETRIC or DIRECTION
  *                       rebuilds the store with the same budget.
  *   THV_OPT_HINT_BYTES  (read only) HBM bytes the hints use, 0 if none.

@@ -201,10 +201,20 @@ def test_parent_hints(rmat16, rmat16u):
     """Parent hints take n x 4 bytes of the budget on a split store of a
     symmetric graph with 4-byte records, place vertices without reading
     their lists, and leave the levels unchanged; switching them off gives
-    the bytes back to the partitions.  8-byte records get no hints."""
-    gw, _ = rmat16
+    the bytes back to the partitions.  A split store with 8-byte records
+    gets no hints; a resident store keeps them beside the lists, for either
+    record size."""
+    gw, ogw = rmat16
     gw.store_config(int(0.25 * gw.m * gw.edge_bytes), 4096)
     assert gw.get_option("hint_bytes") == 0
+    gw.store_config(None, 4096)
+    assert gw.get_option("hint_bytes") == 4 * gw.n
+    assert gw.store_info()["device_edges"] == gw.m
+    for s in (1, 7):
+        lv, st = gw.bfs(s)
+        assert np.array_equal(lv, O.bfs(ogw, s)), s
+        if s == 1:
+            assert st.bottom_up_levels > 0 and st.hint_hits > 0
     g, og = rmat16u
     _say("case: parent hints")
     budget = int(0.25 * g.m * g.edge_bytes)

@@ -281,7 +281,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const bool trace = getenv("THV_TRACE") != nullptr;
   const bool host_share = g.store.host_vertices > 0;
   const double link = g.do_link_gbps * 1e9;
-  const bool hints = host_share && g.store.d_hint.p != nullptr;
+  const bool hints = g.store.d_hint.p != nullptr;
   // probe counters: hint hits, unvisited, unvisited with a hint, words written
   unsigned long long *d_pc = w.aux.as<unsigned long long>() + 4;
   // Cost terms of the split-store rule, measured on MI355X at R-MAT 26:

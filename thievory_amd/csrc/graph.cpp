@@ -154,15 +154,17 @@ static void make_partitions(Graph &g, uint32_t part_edges, std::vector<uint32_t>
   if (g.n == 0) pv.assign({0, 0});
 }
 
-// The bottom-up layout (parent hints + sparsest partitions first) pays off
-// where bottom-up BFS levels run over a split store: a symmetric graph with
-// direction-optimising BFS and 4-byte records (8-byte {dst, weight}
-// records are loaded for SSSP, whose top-down levels want the dense
-// partitions in HBM).  The hints take n x 4 bytes of the budget, so the
-// budget must leave at least as much again for partitions.
+// Parent hints pay off wherever bottom-up BFS levels run: a symmetric graph
+// with direction-optimising BFS.  A resident store keeps them beside the
+// lists (n x 4 bytes more HBM).  A split store takes them from the budget,
+// which must leave at least as much again for partitions, and switches to
+// the bottom-up layout (sparsest partitions first); only with 4-byte
+// records (8-byte {dst, weight} records are loaded for SSSP, whose top-down
+// levels want the dense partitions in HBM).
 bool store_wants_hints(const Graph &g, uint64_t budget) {
-  return g.sym && g.dir_opt && g.parent_hints && g.rec == 4 && g.n > 0 &&
-         budget != UINT64_MAX && budget < g.m * (uint64_t)g.rec && budget >= 2 * 4 * g.n;
+  if (!(g.sym && g.dir_opt && g.parent_hints && g.n > 0)) return false;
+  if (budget >= g.m * (uint64_t)g.rec) return true;  // resident
+  return g.rec == 4 && budget >= 2 * 4 * g.n;
 }
 
 // Options that decide the hints changed: rebuild with the same budget.
@@ -182,7 +184,8 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   s.budget = budget;
   s.hint_reserved = store_wants_hints(g, budget);
   s.d_hint.release();
-  if (s.hint_reserved) budget -= 4 * g.n;  // the hints' share of the budget
+  // a split store's hints come out of its budget
+  if (s.hint_reserved && budget < g.m * (uint64_t)g.rec) budget -= 4 * g.n;
   make_partitions(g, part_edges, s.pv);
   s.P = (uint32_t)(s.pv.size() - 1);
   const uint64_t rec = (uint64_t)g.rec;
@@ -348,7 +351,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   s.d_pres.alloc(s.P ? s.P : 1);
   if (s.P) HIPC(hipMemcpy(s.d_pres.p, s.pres.data(), s.P, hipMemcpyHostToDevice));
   s.configured = true;
-  if (s.hint_reserved && s.host_vertices > 0) build_parent_hints(g);
+  if (s.hint_reserved) build_parent_hints(g);
 }
 
 // PageRank push hubs: count the destinations of every 64th record (host
