@@ -282,7 +282,8 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const bool host_share = g.store.host_vertices > 0;
   const double link = g.do_link_gbps * 1e9;
   const bool hints = g.store.d_hint.p != nullptr;
-  // probe counters: hint hits, unvisited, unvisited with a hint, words written
+  // probe counters: hint hits, unvisited, unvisited with a hint, words
+  // written, candidates written, candidates read
   unsigned long long *d_pc = w.aux.as<unsigned long long>() + 4;
   // Cost terms of the split-store rule, measured on MI355X at R-MAT 26:
   // a short host list scanned in full costs its bytes plus SHORT_NS (a
@@ -293,27 +294,41 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   constexpr double SHORT_NS = 0.6, TD_NS_ENTRY = 0.053, TD_NS_EDGE = 0.019,
                    PROBE_NS_VERTEX = 0.0045, PROBE_FIXED = 30e-6;
   uint64_t visited_edges = 0, prev_nf = 0;
-  double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0;
+  double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0, last_probe_ms = 0;
   uint32_t launches = 0, probes = 0;
   uint64_t probe_bytes = 0;  // the parts that do not come from d_pc
-  HIPC(hipMemsetAsync(d_pc, 0, 4 * 8, s));
+  HIPC(hipMemsetAsync(d_pc, 0, 6 * 8, s));
   // a bottom-up level writes the values of the vertices it places and sums
   // the next frontier (c_next), so the next level needs no count pass
   LevelCtr *c_next = d_ctr + 4;
   bool fused_next = false;
+  // the first probe sweeps every vertex; it and every later one write the
+  // vertices still unvisited after their hint test (candidates), and a
+  // later probe reads only those (cand: the buffer holding the last list)
+  int cand = -1;
+  const uint64_t ntiles = (g.n + TILE - 1) / TILE;
   auto probe = [&](const uint32_t *hint, uint32_t *next, uint32_t level_value) {
     HIPC(hipMemsetAsync(c_next, 0, sizeof(LevelCtr), s));
-    launch_bu_probe(g, w.qs, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint, D,
+    const int out = cand < 0 ? 0 : cand ^ 1;
+    QueueSet &q = w.qs;
+    launch_bu_probe(g, q, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint, D,
                     c_short, c_long, d_pc, s, w.ev[2], w.ev[3], c_next, w.val.as<uint32_t>(),
-                    level_value);
+                    level_value, cand < 0 ? nullptr : q.clist[cand].as<uint32_t>(),
+                    cand < 0 ? nullptr : q.ccnt[cand].as<uint32_t>(), q.clist[out].as<uint32_t>(),
+                    q.ccnt[out].as<uint32_t>());
+    // a sweep reads the visited and residency words and writes the tile
+    // tables of both sets, a list probe reads its regions' candidate
+    // counts (the candidates themselves are counted on the device); both
+    // write the new candidate counts
+    probe_bytes += (cand < 0 ? 2 * 4 * words + 2 * 24 * ntiles : 4 * 4 * ntiles) + 4 * 4 * ntiles;
     probes++;
-    // visited and residency words, tile tables of both sets
-    probe_bytes += 2 * 4 * words + 2 * 24 * ((g.n + TILE - 1) / TILE);
+    cand = out;
   };
   auto probe_done = [&]() {  // after the stream has drained past the probe
     float pm = 0;
     HIPC(hipEventElapsedTime(&pm, w.ev[2], w.ev[3]));
     probe_ms += pm;
+    last_probe_ms = pm;
     // short lists: an address each, their HBM records, and a frontier word
     // per record (records read from host memory count as host bytes)
     const LevelCtr &ps = w.h_ctr[3];
@@ -349,12 +364,13 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       fprintf(stderr,
               "[thv] bfs L=%u %s nf=%llu mf=%llu front_host_edges=%llu est_td_ms=%.3f "
               "est_bu_ms=%.3f hint_hits_total=%llu short=%llu short_host_edges=%llu long=%llu "
-              "long_host=%llu read_host=%llu read_hbm=%llu kernel_ms=%.3f\n",
+              "long_host=%llu read_host=%llu read_hbm=%llu kernel_ms=%.3f probe_ms=%.3f\n",
               p.L, p.bu ? "bottom-up" : "top-down", (unsigned long long)p.nf,
               (unsigned long long)p.mf, p.h.qw[0], p.td_cost * 1e3, p.bu_cost * 1e3,
               p.hinted ? w.h_aux[1] : 0ull,
               hs.qn[0] + hs.qn[1], hs.qw[0], hb.qn[0] + hb.qn[1], hb.qn[0],
-              p.bu ? hs.qw[0] + hb.rd[0] : p.h.qw[0], p.bu ? hs.qw[1] + hb.rd[1] : p.h.qw[1], ms);
+              p.bu ? hs.qw[0] + hb.rd[0] : p.h.qw[0], p.bu ? hs.qw[1] + hb.rd[1] : p.h.qw[1], ms,
+              p.bu || p.probed ? last_probe_ms : 0.0);
     }
     if (st) {
       st->edges_traversed += p.mf;
@@ -493,7 +509,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     st->expand_launches = launches;
     st->bottom_up_ms = bottom_up_ms;
     if (probes) {
-      HIPC(hipMemcpy(w.h_aux, d_pc, 4 * 8, hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(w.h_aux, d_pc, 6 * 8, hipMemcpyDeviceToHost));
       const unsigned long long *pc = w.h_aux;
       st->hint_hits = pc[0];
       st->probe_ms = probe_ms;
@@ -501,9 +517,10 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       // per unvisited vertex: its hint word (when hints are on) and, when
       // it has one, the frontier word it names, and its degree; the value
       // of each vertex a hint placed; 12 bytes per bitmap word updated
-      // (visited written, next read and written)
+      // (visited written, next read and written); 4 bytes per candidate
+      // written, 12 per candidate read (its id, visited and residency words)
       st->probe_hbm_bytes = probe_bytes + (hints ? 4 * pc[1] + 4 * pc[2] : 0) + 4 * pc[1] +
-                            4 * pc[0] + 12 * pc[3];
+                            4 * pc[0] + 12 * pc[3] + 4 * pc[4] + 12 * pc[5];
       st->hbm_bytes += st->probe_hbm_bytes;
     }
   }

@@ -374,7 +374,8 @@ struct ProbeArgs {
   uint32_t *tcnt[2];
   unsigned long long *tdeg[2];
   unsigned long long *ssum[2];
-  unsigned long long *cnt;  // [0] hint hits, [1] unvisited, [2] with a hint, [3] words written
+  unsigned long long *cnt;  // [0] hint hits, [1] unvisited, [2] with a hint, [3] words
+                            // written, [4] candidates written, [5] candidates read
   uint32_t *slist;          // short lists: wave (tile, w) writes from slot tile*TILE + w*SUB
   uint32_t *scnt;           //   and their count at [tile * 4 + w]
   uint32_t *llist, *lcnt;   // long lists (degree > D), same layout
@@ -385,9 +386,47 @@ struct ProbeArgs {
   uint32_t *val;
   uint32_t level_value;
   LevelCtr *nctr;
+  // candidates: every vertex still unvisited after the hint test goes to
+  // cout (per wave region, in vertex order; count at ccout[region]).  A
+  // later probe of the same traversal reads that list (cin/ccin) instead of
+  // sweeping every vertex: the unvisited set only shrinks.
+  const uint32_t *cin, *ccin;
+  uint32_t *cout, *ccout;
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
+
+// The probes' counters: one atomic per counter and block (the grid is a
+// few blocks per CU).
+__device__ __forceinline__ void probe_flush(const ProbeArgs &a, unsigned long long (*s)[10],
+                                            uint32_t wv, uint32_t lane, unsigned long long hits,
+                                            unsigned long long nun, unsigned long long nhint,
+                                            unsigned long long nwords, const uint32_t fn[2],
+                                            const unsigned long long fe[2],
+                                            unsigned long long ncout, unsigned long long ncin) {
+  const unsigned long long c10[10] = {wave_sum(hits),  wave_sum(nun),
+                                      wave_sum(nhint), wave_sum(nwords),
+                                      wave_sum((unsigned long long)fn[0]),
+                                      wave_sum((unsigned long long)fn[1]),
+                                      wave_sum(fe[0]), wave_sum(fe[1]),
+                                      wave_sum(ncout), wave_sum(ncin)};
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 10; i++) s[wv][i] = c10[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    unsigned long long t = 0;
+    for (int w = 0; w < CBS / 64; w++) t += s[w][threadIdx.x];
+    if (t) {
+      const uint32_t i = threadIdx.x;
+      if (i < 4) atomicAdd(&a.cnt[i], t);
+      else if (i < 6) atomicAdd(&a.nctr->qn[i - 4], t);
+      else if (i < 8) atomicAdd(&a.nctr->qw[i - 6], t);
+      else atomicAdd(&a.cnt[i - 4], t);
+    }
+  }
+}
 
 // Bottom-up probe, one wave per 1024-vertex sub-tile:
 //   1. visited and residency words (lanes 0-31, one load each);
@@ -397,15 +436,18 @@ constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
 //      that loaded them;
 //   3. degrees of the rest.  Lists of 1..D records go to the wave's slots
 //      of the short-list array, in vertex order, for k_bu_short; longer
-//      ones to the long-list array, for the early-exit kernel.
+//      ones to the long-list array, for the early-exit kernel; every
+//      unvisited vertex without a hint hit to the candidate list, which a
+//      later probe of the traversal reads instead of sweeping
+//      (k_bu_probe_list).
 //   Counts: tile set 0 holds the short lists (entries, records), tile set
 //   1 the long unvisited lists (degree > D) left for the early-exit kernel,
 //   both split by residency.
 __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
-  __shared__ unsigned long long s[CBS / 64][8];
+  __shared__ unsigned long long s[CBS / 64][10];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = lanemask_lt();
-  unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0;
+  unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0, ncout = 0;
   unsigned long long fe[2] = {0, 0};  // next frontier: edges (host, HBM)
   uint32_t fn[2] = {0, 0};            //   and entries
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
@@ -424,6 +466,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
       if (lane == 0) {
         a.scnt[tile * (CBS / 64) + wv] = 0;
         a.lcnt[tile * (CBS / 64) + wv] = 0;
+        if (a.ccout) a.ccout[tile * (CBS / 64) + wv] = 0;
       }
       if (threadIdx.x < 8) {
         const uint32_t set = threadIdx.x >> 2, i = threadIdx.x & 3;
@@ -488,7 +531,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     const uint32_t dlo = a.hint ? 2u : 1u;
     uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
     unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
-    uint32_t ns = 0, nl = 0;
+    uint32_t ns = 0, nl = 0, nc = 0;
 #pragma unroll
     for (uint32_t r = 0; r < ROUNDS; r++) {
       const uint32_t q = (dev >> r) & 1u;
@@ -500,6 +543,12 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
       if (lg) a.llist[base + nl + __popcll(bl & lt)] = (uint32_t)(base + r * 64 + lane);
       ns += __popcll(b);
       nl += __popcll(bl);
+      if (a.cout) {
+        const bool c = (un >> r) & 1u;
+        const unsigned long long bc = __ballot(c);
+        if (c) a.cout[base + nc + __popcll(bc & lt)] = (uint32_t)(base + r * 64 + lane);
+        nc += __popcll(bc);
+      }
       if (d[r] < dlo) continue;
       const uint32_t k = (sh ? 0 : 2) + q;
       xc[k] += 1;
@@ -508,6 +557,10 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     if (lane == 0) {
       a.scnt[tile * (CBS / 64) + wv] = ns;
       a.lcnt[tile * (CBS / 64) + wv] = nl;
+      if (a.ccout) {
+        a.ccout[tile * (CBS / 64) + wv] = nc;
+        ncout += nc;
+      }
     }
     hits += __popc(m);  // hint hits
     if (owner && m) {
@@ -536,27 +589,159 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     }
     __syncthreads();
   }
-  // one atomic per counter and block (the grid is a few blocks per CU)
-  const unsigned long long c8[8] = {wave_sum(hits),  wave_sum(nun),
-                                    wave_sum(nhint), wave_sum(nwords),
-                                    wave_sum((unsigned long long)fn[0]),
-                                    wave_sum((unsigned long long)fn[1]),
-                                    wave_sum(fe[0]), wave_sum(fe[1])};
-  if (lane == 0) {
+  probe_flush(a, s, wv, lane, hits, nun, nhint, nwords, fn, fe, ncout, 0);
+}
+
+// Lanes [lo, hi) of a round whose first flat index is t0, for the run of
+// flat indices [o, o + c).
+__device__ __forceinline__ unsigned long long run_lanes(uint32_t o, uint32_t c, uint32_t t0) {
+  const int64_t lo = (int64_t)o - t0, hi = (int64_t)o + c - t0;
+  if (hi <= 0 || lo >= 64 || c == 0) return 0ull;
+  const uint32_t l = lo > 0 ? (uint32_t)lo : 0u, h = hi < 64 ? (uint32_t)hi : 64u;
+  const unsigned long long upto = h == 64 ? ~0ull : (1ull << h) - 1ull;
+  return upto & ~((1ull << l) - 1ull);
+}
+
+// List-mode probe: one wave per group of LR consecutive regions, over the
+// candidates an earlier probe of the traversal wrote for them (vertices
+// unvisited after its hint test; the unvisited set only shrinks).  The
+// group's candidates are spread over the wave's lanes, 64 per round, so a
+// level with few unvisited vertices costs a round or two per wave instead
+// of a sweep of every vertex.  Per candidate: still unvisited (its visited
+// bit) and its residency, then the sweep's hint test, degree and list
+// classification.  Outputs are the sweep's: short lists, long lists and
+// new candidates in each region's slots (lane j keeps region j's running
+// counts), hit words per region (collected in LDS, written by this wave,
+// which owns the group's words), and the segment sums of both sets.  No
+// tile tables: after a probe nothing reads them.
+constexpr uint32_t LR = 8;  // regions per group
+static_assert(SEG * (CBS / 64) % LR == 0, "a group lies in one segment");
+__global__ void __launch_bounds__(CBS) k_bu_probe_list(ProbeArgs a) {
+  __shared__ uint32_t hw[CBS / 64][LR * (SUB / 32)];
+  __shared__ unsigned long long s[CBS / 64][10];
+  constexpr uint32_t VNONE = 0xFFFFFFFFu;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long lt = lanemask_lt();
+  const uint32_t nreg = a.ntiles * (CBS / 64), ngroups = (nreg + LR - 1) / LR;
+  const uint32_t nwaves = gridDim.x * (CBS / 64);
+  const uint32_t dlo = a.hint ? 2u : 1u;
+  unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0, ncin = 0, ncout = 0;
+  unsigned long long fe[2] = {0, 0};
+  uint32_t fn[2] = {0, 0};
+  for (uint32_t gi = blockIdx.x * (CBS / 64) + wv; gi < ngroups; gi += nwaves) {
+    const uint32_t r0 = gi * LR;
+    // lane j < LR: region r0 + j's candidates [oj, oj + cj) in the group
+    const bool rl = lane < LR && r0 + lane < nreg;
+    const uint32_t cj = rl ? a.ccin[r0 + lane] : 0u;
+    const uint32_t incl = wave_incl_scan(cj), oj = incl - cj;
+    const uint32_t T = (uint32_t)__shfl((int)incl, 63, 64);
+    if (T == 0) {
+      if (rl) {
+        a.scnt[r0 + lane] = 0;
+        a.lcnt[r0 + lane] = 0;
+        if (a.ccout) a.ccout[r0 + lane] = 0;
+      }
+      continue;
+    }
+    if (lane == 0) ncin += T;
+    for (uint32_t i = lane; i < LR * (SUB / 32); i += 64) hw[wv][i] = 0;
+    uint32_t ns = 0, nl = 0, nc = 0;          // lane j: region r0 + j's counts so far
+    uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
+    unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
+    for (uint32_t t0 = 0; t0 < T; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      // the lane's region: the last j whose run starts at or before t
+      uint32_t j = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) s[wv][i] = c8[i];
-  }
-  __syncthreads();
-  if (threadIdx.x < 8) {
-    unsigned long long t = 0;
-    for (int w = 0; w < CBS / 64; w++) t += s[w][threadIdx.x];
-    if (t) {
-      const uint32_t i = threadIdx.x;
-      if (i < 4) atomicAdd(&a.cnt[i], t);
-      else if (i < 6) atomicAdd(&a.nctr->qn[i - 4], t);
-      else atomicAdd(&a.nctr->qw[i - 6], t);
+      for (uint32_t q = 1; q < LR; q++)
+        if ((uint32_t)__shfl((int)oj, q, 64) <= t) j = q;
+      const uint32_t oJ = (uint32_t)__shfl((int)oj, j, 64);
+      const uint32_t cJ = (uint32_t)__shfl((int)cj, j, 64);
+      const uint64_t rbase = (uint64_t)(r0 + j) * SUB;
+      const uint32_t v = t < T ? a.cin[rbase + (t - oJ)] : VNONE;
+      const uint32_t vw = v != VNONE ? a.vis[v >> 5] : 0xFFFFFFFFu;
+      const uint32_t rw = v != VNONE ? a.rbit[v >> 5] : 0u;
+      const uint32_t bit = 1u << (v & 31);
+      bool un = !(vw & bit);
+      const uint32_t q = (rw & bit) ? 1u : 0u;
+      nun += un;
+      bool hit = false;
+      if (a.hint) {
+        const uint32_t h = un ? a.hint[v] : NO_HINT;
+        nhint += h != NO_HINT;
+        const uint32_t fw = a.front[h != NO_HINT ? h >> 5 : 0];
+        hit = h != NO_HINT && ((fw >> (h & 31)) & 1u);
+        if (hit) {
+          const uint32_t o = (uint32_t)(v - rbase);
+          atomicOr(&hw[wv][j * (SUB / 32) + (o >> 5)], bit);
+        }
+      }
+      // with the bookkeeping on, a hit's degree is the next frontier's
+      uint32_t d = un && (!hit || a.nctr) ? a.deg[v] : 0u;
+      if (hit) {
+        if (a.nctr) {
+          a.val[v] = a.level_value;
+          fn[q] += 1;
+          fe[q] += d;
+        }
+        d = 0;
+        un = false;
+      }
+      const bool sh = un && d >= dlo && d <= a.D;
+      const bool lg = un && d >= dlo && d > a.D;
+      const unsigned long long mj = run_lanes(oJ, cJ, t0);
+      const unsigned long long bs = __ballot(sh), bl = __ballot(lg), bc = __ballot(un);
+      const uint32_t ps = (uint32_t)__shfl((int)ns, j, 64), pl = (uint32_t)__shfl((int)nl, j, 64);
+      const uint32_t pc = (uint32_t)__shfl((int)nc, j, 64);
+      if (sh) a.slist[rbase + ps + __popcll(bs & lt & mj)] = v;
+      if (lg) a.llist[rbase + pl + __popcll(bl & lt & mj)] = v;
+      if (un && a.cout) a.cout[rbase + pc + __popcll(bc & lt & mj)] = v;
+      if (lane < LR) {  // this round's lanes of region r0 + lane
+        const unsigned long long mo = run_lanes(oj, cj, t0);
+        ns += __popcll(bs & mo);
+        nl += __popcll(bl & mo);
+        nc += __popcll(bc & mo);
+      }
+      if (un && d >= dlo) {
+        const uint32_t k = (sh ? 0 : 2) + q;
+        xc[k] += 1;
+        xe[k] += d;
+      }
+    }
+    if (rl) {
+      a.scnt[r0 + lane] = ns;
+      a.lcnt[r0 + lane] = nl;
+      if (a.ccout) {
+        a.ccout[r0 + lane] = nc;
+        ncout += nc;
+      }
+    }
+    // the group's hit words (its regions are consecutive, so are they)
+    const uint64_t w0 = (uint64_t)r0 * (SUB / 32);
+    for (uint32_t i = lane; i < LR * (SUB / 32); i += 64) {
+      const uint32_t m = hw[wv][i];
+      if (m) {
+        a.vis[w0 + i] |= m;
+        a.next[w0 + i] |= m;
+        nwords++;
+        hits += __popc(m);
+      }
+    }
+    // the group's segment sums: s[set * 4 + i], i = 0, 1 entries (host,
+    // HBM), 2, 3 edges
+    const uint32_t seg = (r0 / (CBS / 64)) / SEG;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const unsigned long long c = wave_sum((unsigned long long)xc[k]);
+      const unsigned long long e = wave_sum(xe[k]);
+      if (lane == 0) {
+        const uint32_t set = (uint32_t)k >> 1, i = (uint32_t)k & 1u;
+        if (c) atomicAdd(&a.ssum[set][seg * 4 + i], c);
+        if (e) atomicAdd(&a.ssum[set][seg * 4 + 2 + i], e);
+      }
     }
   }
+  probe_flush(a, s, wv, lane, hits, nun, nhint, nwords, fn, fe, ncout, ncin);
 }
 
 struct ShortArgs {
@@ -733,7 +918,9 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
 void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis,
                      uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s, hipEvent_t t0,
-                     hipEvent_t t1, LevelCtr *nctr, uint32_t *val, uint32_t level_value) {
+                     hipEvent_t t1, LevelCtr *nctr, uint32_t *val, uint32_t level_value,
+                     const uint32_t *cin, const uint32_t *ccin, uint32_t *cout,
+                     uint32_t *ccout) {
   ProbeArgs p;
   memset(&p, 0, sizeof p);
   p.front = front;
@@ -762,9 +949,22 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   p.val = val;
   p.level_value = level_value;
   if (nctr && !val) fail(THV_ESTATE, "next-frontier bookkeeping needs the value array");
+  if ((cin == nullptr) != (ccin == nullptr) || (cout == nullptr) != (ccout == nullptr) ||
+      (cin && cin == cout))
+    fail(THV_ESTATE, "probe candidate lists: both arrays, and distinct input and output");
+  p.cin = cin;
+  p.ccin = ccin;
+  p.cout = cout;
+  p.ccout = ccout;
   const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
   if (t0) HIPC(hipEventRecord(t0, s));
-  hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);
+  if (cin) {
+    const uint32_t groups = (uint32_t)div_up((uint64_t)p.ntiles * (CBS / 64), LR);
+    const uint32_t lgrid = std::min<uint32_t>((uint32_t)div_up(groups, CBS / 64), 64u * g.cus);
+    hipLaunchKernelGGL(k_bu_probe_list, dim3(lgrid), dim3(CBS), 0, s, p);
+  } else {
+    hipLaunchKernelGGL(k_bu_probe, dim3(grid), dim3(CBS), 0, s, p);
+  }
   if (D > 0) {
     ShortArgs sa;
     sa.slist = p.slist;

@@ -400,6 +400,10 @@ void QueueSet::ensure(const Graph &g, uint32_t chunk) {
     llist.alloc(tiles * TILE * 4);
     lcnt.alloc(tiles * 4 * 4);
     loff.alloc((tiles * 4 + 1) * 4);
+    for (int k = 0; k < 2; k++) {
+      clist[k].alloc(tiles * TILE * 4);
+      ccnt[k].alloc(tiles * 4 * 4);
+    }
   }
   for (int k = 0; k < 2; k++) {
     if (tcnt[k].bytes < tiles * 2 * 4) {
@@ -431,12 +435,12 @@ void workspace_ensure(Graph &g) {
   const uint64_t words = bitmap_words(g.n);
   for (auto &f : w.front) f.alloc(words * 4);
   w.vis.alloc(words * 4);
-  w.aux.alloc(8 * sizeof(unsigned long long));
+  w.aux.alloc(16 * sizeof(unsigned long long));
   w.val.alloc((g.n + 1) * 4);
   if (!w.h_ctr)
     HIPC(hipHostMalloc((void **)&w.h_ctr, CTR_SLOTS * sizeof(LevelCtr), hipHostMallocDefault));
   if (!w.h_aux)
-    HIPC(hipHostMalloc((void **)&w.h_aux, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    HIPC(hipHostMalloc((void **)&w.h_aux, 16 * sizeof(unsigned long long), hipHostMallocDefault));
   for (auto &e : w.ev)
     if (!e) HIPC(hipEventCreate(&e));
   w.n = g.n;

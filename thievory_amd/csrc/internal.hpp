@@ -135,6 +135,7 @@ struct QueueSet {
   DevMem slist, scnt;         // bottom-up probe: short lists per wave region
   DevMem llist, lcnt;         //   and long lists (for the early-exit kernel)
   DevMem loff;                //   with their regions' exclusive bases (+ total)
+  DevMem clist[2], ccnt[2];   //   and candidates for the traversal's next probe
   void ensure(const Graph &g, uint32_t C);
 };
 
@@ -145,7 +146,7 @@ struct Workspace {
   DevMem vis;                 // BFS visited bitmap
   DevMem val;                 // u32 values (BFS / CC / SSSP)
   DevMem f0, f1, f2;          // PR arrays (fp32)
-  DevMem aux;                 // misc device counters (8 x u64)
+  DevMem aux;                 // misc device counters (16 x u64)
   LevelCtr *h_ctr = nullptr;  // pinned readback
   unsigned long long *h_aux = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // [2], [3]: probe
@@ -275,7 +276,9 @@ void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vi
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s,
                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
                      LevelCtr *nctr = nullptr, uint32_t *val = nullptr,
-                     uint32_t level_value = 0);
+                     uint32_t level_value = 0, const uint32_t *cin = nullptr,
+                     const uint32_t *ccin = nullptr, uint32_t *cout = nullptr,
+                     uint32_t *ccout = nullptr);
 
 struct BottomUpArgs {
   const uint32_t *llist, *loff;  // the probe's long lists, per wave region,
