@@ -175,10 +175,11 @@ def measured_traffic(workload, kernel):
 def bfs_roofline(stats, workload=None):
     """Roofline of the BFS steps.  Kernel view: the traversal kernels'
     algorithmic bytes over their HIP-event time -- the bottom-up probe
-    (sweep + short-list scan, one event window) against HBM (its reads are
-    the visited / residency / hint / degree words, short lists and frontier
-    words, all in HBM), the top-down expansion and the early-exit bottom-up
-    window against the host link.
+    (the first level's sweep or a later level's candidate-list probe, then
+    the short-list scan: one event window) against HBM (its reads are the
+    visited / residency / hint / degree words, candidates, short lists and
+    frontier words, all in HBM), the top-down expansion and the early-exit
+    bottom-up window against the host link.
     The dominant kernel (most time) is the headline.  Mixed views: T_roof =
     max(HBM bytes / 8 TB/s, host bytes / 64 GB/s) against the step time,
     once with the bytes this run read and once with the bytes a
@@ -186,11 +187,11 @@ def bfs_roofline(stats, workload=None):
     med = statistics.median(s.ms for s in stats)
     kern = {}
     for name, ms_key, launches, nbytes, bound in (
-            ("k_bu_probe+k_bu_short", "probe_ms", lambda s: s.probe_launches,
+            ("k_bu_probe|k_bu_probe_list+k_bu_short", "probe_ms", lambda s: s.probe_launches,
              lambda s: s.probe_hbm_bytes, "hbm"),
             ("k_expand<BFS>", "expand_ms", lambda s: s.expand_launches,
              lambda s: s.host_bytes - s.bottom_up_host_bytes, "host-link"),
-            ("bottom-up window (long-list compaction + k_bottom_up)", "bottom_up_ms",
+            ("bottom-up window (k_bottom_up)", "bottom_up_ms",
              lambda s: s.bottom_up_levels, lambda s: s.bottom_up_host_bytes, "host-link")):
         t = sum(getattr(s, ms_key) for s in stats)
         n = sum(launches(s) for s in stats)
