@@ -161,6 +161,47 @@ def test_bfs_direction_optimizing(rmat16, frac):
     g.set_option("do_alpha", 14.0)
 
 
+def _beads(k, s):
+    """k cliques of s vertices in a row, clique b's last vertex joined to
+    clique b+1's first: from vertex 0 the frontier alternates between one
+    vertex and a whole clique."""
+    pairs = []
+    for b in range(k):
+        c = np.arange(b * s, (b + 1) * s)
+        u, v = np.meshgrid(c, c)
+        keep = u != v
+        pairs.append(np.stack([u[keep], v[keep]], 1))
+        if b + 1 < k:
+            pairs.append(np.array([[b * s + s - 1, (b + 1) * s], [(b + 1) * s, b * s + s - 1]]))
+    p = np.concatenate(pairs)
+    p = p[np.lexsort((p[:, 1], p[:, 0]))]
+    off = np.zeros(k * s + 1, np.uint64)
+    np.add.at(off, p[:, 0] + 1, 1)
+    return O.CSR(np.cumsum(off).astype(np.uint64), p[:, 1].astype(np.uint32))
+
+
+@pytest.mark.parametrize("frac", [1.0, 0.5])
+def test_bfs_beads_alternating_directions(frac):
+    """Levels alternate bottom-up (a clique's worth of frontier) and
+    top-down (one bridge vertex), so every probe after the first reads the
+    candidates an earlier probe left, across top-down levels in between:
+    the levels equal the oracle's."""
+    og = _beads(60, 24)
+    g = T.Graph.from_arrays(og.off, og.adj)
+    g.set_option("symmetric", 1)
+    g.set_option("do_alpha", 1000.0)   # bottom-up as soon as the frontier grows
+    g.set_option("do_beta", 1.0)       # top-down as soon as it shrinks
+    g.set_option("do_link_gbps", 0.001)
+    g.store_config(int(frac * og.m * 4), 64)
+    _say("case: beads", frac)
+    for s in (0, 700, og.n - 1):
+        lv, st = g.bfs(s)
+        assert np.array_equal(lv, O.bfs(og, s)), s
+        if frac == 1.0 and s == 0:
+            assert st.bottom_up_levels >= 20, st.bottom_up_levels
+            assert st.iterations - st.bottom_up_levels >= 20, st.iterations
+
+
 @pytest.fixture(scope="module")
 def rmat16u():
     """Symmetric R-MAT 16 with 4-byte records: the bottom-up layout (parent
