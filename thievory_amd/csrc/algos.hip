@@ -338,15 +338,25 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   // Level bookkeeping, run after the next level's count has synchronised
   // the stream (one host round trip per level instead of two).
   struct LevelRec {
-    bool valid = false, bu = false, probed = false, hinted = false;
+    // sure: bottom-up without a readback of the probe's counts (see below)
+    bool valid = false, bu = false, probed = false, hinted = false, sure = false;
     uint32_t L = 0;
     uint64_t nf = 0, mf = 0;
     LevelCtr h;
     double td_cost = 0, bu_cost = 0;
   } pend;
-  auto finish_level = [&](const LevelRec &p) {
+  // the bottom-up cost of the last level if it was probed and ran
+  // bottom-up, else negative
+  double prev_bu_cost = -1;
+  auto short_long_cost = [&](const LevelCtr &pl, const LevelCtr &ps) {
+    return (double)ps.qw[0] * rec / link + (double)ps.qn[0] * SHORT_NS * 1e-9 +
+           (double)pl.qn[0] * g.do_vertex_ns * 1e-9;
+  };
+  auto finish_level = [&](LevelRec &p) {
     if (!p.valid) return;
-    if (p.bu && !p.probed) probe_done();
+    if (p.bu && (!p.probed || p.sure)) probe_done();
+    if (p.sure) p.bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
+    prev_bu_cost = p.bu && p.probed ? p.bu_cost : -1;
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
     LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
@@ -420,7 +430,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
     // probed: the bottom-up probe ran (tile set 0 now holds the short
     // unvisited lists, so a top-down level needs a fresh frontier count)
-    bool probed = false, hinted = false;
+    bool probed = false, hinted = false, sure = false;
     double td_cost = 0, bu_cost = 0;
     HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
     const uint32_t *hint = hints ? g.store.d_hint.as<uint32_t>() : nullptr;
@@ -438,16 +448,25 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       bu = false;
       if (td_cost > probe_cost) {
         probe(hint, next, L + 2);
-        HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
-        if (trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
-        probe_done();
-        const LevelCtr &pl = w.h_ctr[2], &ps = w.h_ctr[3];
-        bu_cost = (double)ps.qw[0] * rec / link + (double)ps.qn[0] * SHORT_NS * 1e-9 +
-                  (double)pl.qn[0] * g.do_vertex_ns * 1e-9;
-        bu = bu_cost < td_cost;
         probed = true;
         hinted = hints;
+        // the short and long unvisited host lists of this level are subsets
+        // of the last level's (the unvisited set only shrinks, degrees are
+        // fixed), so its bottom-up cost is at most the last level's: if that
+        // is below this level's top-down cost, bottom-up wins without the
+        // readback (the counts are read with the level's bookkeeping)
+        if (prev_bu_cost >= 0 && prev_bu_cost < td_cost) {
+          bu = sure = true;
+        } else {
+          HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost,
+                              s));
+          if (trace && hints)
+            HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
+          HIPC(hipStreamSynchronize(s));
+          probe_done();
+          bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
+          bu = bu_cost < td_cost;
+        }
       }
     } else {
       // everything in HBM: the edge-ratio rule of Beamer et al.
@@ -489,11 +508,13 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       launch_bottom_up(g, b, s);
       HIPC(hipEventRecord(w.ev[1], s));
       HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+      if (sure && trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
     }
     pend.valid = true;
     pend.bu = bu;
     pend.probed = probed;
     pend.hinted = hinted;
+    pend.sure = sure;
     pend.L = L;
     pend.nf = nf;
     pend.mf = mf;
