@@ -293,6 +293,16 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   // PROBE_NS_VERTEX per vertex plus PROBE_FIXED (launches and readback).
   constexpr double SHORT_NS = 0.6, TD_NS_ENTRY = 0.053, TD_NS_EDGE = 0.019,
                    PROBE_NS_VERTEX = 0.0045, PROBE_FIXED = 30e-6;
+  // A top-down level also pays its compaction's passes over every tile
+  // (TD_FIXED plus TD_NS_TILE_ALL per tile) and TD_NS_TILE per tile that
+  // holds frontier vertices; a later probe (candidate list) costs
+  // LIST_FIXED for its launches plus LIST_NS_CAND per candidate.  Also
+  // measured at R-MAT 26: a 78-vertex top-down level 45 us, a 25K-vertex
+  // one 170 us, a list probe level over 25K candidates 60 us.
+  constexpr double TD_FIXED = 25e-6, TD_NS_TILE_ALL = 1.2, TD_NS_TILE = 9.7,
+                   LIST_FIXED = 60e-6, LIST_NS_CAND = 0.03;
+  const double ntiles_d = (double)((g.n + TILE - 1) / TILE);
+  uint64_t last_cand = 0;  // candidates the latest probe left
   uint64_t visited_edges = 0, prev_nf = 0;
   double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0, last_probe_ms = 0;
   uint32_t launches = 0, probes = 0;
@@ -425,6 +435,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     pend.valid = false;
     const LevelCtr h = w.h_ctr[L & 1];
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
+    if (fused) last_cand = h.cand;
     if (nf == 0) break;
     visited_edges += mf;
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
@@ -442,9 +453,14 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       // read per long unvisited host list.  The probe (hints, short lists,
       // counts of the unvisited set) runs only when top-down costs more
       // than the probe sweep itself.
-      td_cost = (double)h.qw[0] * rec / link +
-                ((double)nf * TD_NS_ENTRY + (double)mf * TD_NS_EDGE) * 1e-9;
-      const double probe_cost = (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED;
+      // tiles with frontier vertices: nf vertices over the tiles at random
+      const double tiles_work = ntiles_d * -std::expm1(-(double)nf / ntiles_d);
+      td_cost = (double)h.qw[0] * rec / link + TD_FIXED +
+                ((double)nf * TD_NS_ENTRY + (double)mf * TD_NS_EDGE +
+                 ntiles_d * TD_NS_TILE_ALL + tiles_work * TD_NS_TILE) * 1e-9;
+      const double probe_cost =
+          cand < 0 ? (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED
+                   : (double)last_cand * LIST_NS_CAND * 1e-9 + LIST_FIXED;
       bu = false;
       if (td_cost > probe_cost) {
         probe(hint, next, L + 2);
@@ -460,12 +476,14 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         } else {
           HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost,
                               s));
+          HIPC(hipMemcpyAsync(&w.h_ctr[4], c_next, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
           if (trace && hints)
             HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
           HIPC(hipStreamSynchronize(s));
           probe_done();
           bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
           bu = bu_cost < td_cost;
+          last_cand = w.h_ctr[4].cand;
         }
       }
     } else {

@@ -424,6 +424,7 @@ __device__ __forceinline__ void probe_flush(const ProbeArgs &a, unsigned long lo
       else if (i < 6) atomicAdd(&a.nctr->qn[i - 4], t);
       else if (i < 8) atomicAdd(&a.nctr->qw[i - 6], t);
       else atomicAdd(&a.cnt[i - 4], t);
+      if (i == 8 && a.nctr) atomicAdd(&a.nctr->cand, t);
     }
   }
 }

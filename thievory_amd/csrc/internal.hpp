@@ -85,9 +85,9 @@ struct Store {
   DevMem d_pv, d_pres, d_eptr;
   DevMem d_rbit;                     // n bits: vertex's edges are in HBM
   DevMem d_vpart;                    // n x u32: partition of each vertex
-  // Parent hints (symmetric graphs, split store): n x u32, for a vertex
-  // with edges its neighbour of largest degree, else NO_HINT.  Their bytes
-  // come out of the budget.
+  // Parent hints (symmetric graphs): n x u32, for a vertex with edges its
+  // neighbour of largest degree, else NO_HINT.  A split store takes their
+  // bytes out of the budget; a resident store holds them beside the lists.
   bool hint_reserved = false;
   DevMem d_hint;
 };
@@ -105,6 +105,7 @@ struct LevelCtr {
   unsigned int touched[2];    // partitions with >=1 active vertex
   unsigned int pad[2];
   unsigned long long rd[2];   // bottom-up BFS: edge records read (host, HBM)
+  unsigned long long cand;    // bottom-up BFS: candidates the level's probe left
 };
 
 // [0], [1]: alternating levels; [2]: bottom-up early-exit queue; [3]:
