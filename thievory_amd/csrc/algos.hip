@@ -303,6 +303,14 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
                    LIST_FIXED = 60e-6, LIST_NS_CAND = 0.03;
   const double ntiles_d = (double)((g.n + TILE - 1) / TILE);
   uint64_t last_cand = 0;  // candidates the latest probe left
+  // HBM side of a top-down level: tiles with frontier vertices taken as nf
+  // vertices over the tiles at random
+  auto td_hbm_cost = [&](uint64_t nf, uint64_t mf) {
+    const double tiles_work = ntiles_d * -std::expm1(-(double)nf / ntiles_d);
+    return TD_FIXED + ((double)nf * TD_NS_ENTRY + (double)mf * TD_NS_EDGE +
+                       ntiles_d * TD_NS_TILE_ALL + tiles_work * TD_NS_TILE) * 1e-9;
+  };
+  auto list_cost = [&]() { return (double)last_cand * LIST_NS_CAND * 1e-9 + LIST_FIXED; };
   uint64_t visited_edges = 0, prev_nf = 0;
   double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0, last_probe_ms = 0;
   uint32_t launches = 0, probes = 0;
@@ -453,14 +461,9 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       // read per long unvisited host list.  The probe (hints, short lists,
       // counts of the unvisited set) runs only when top-down costs more
       // than the probe sweep itself.
-      // tiles with frontier vertices: nf vertices over the tiles at random
-      const double tiles_work = ntiles_d * -std::expm1(-(double)nf / ntiles_d);
-      td_cost = (double)h.qw[0] * rec / link + TD_FIXED +
-                ((double)nf * TD_NS_ENTRY + (double)mf * TD_NS_EDGE +
-                 ntiles_d * TD_NS_TILE_ALL + tiles_work * TD_NS_TILE) * 1e-9;
+      td_cost = (double)h.qw[0] * rec / link + td_hbm_cost(nf, mf);
       const double probe_cost =
-          cand < 0 ? (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED
-                   : (double)last_cand * LIST_NS_CAND * 1e-9 + LIST_FIXED;
+          cand < 0 ? (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED : list_cost();
       bu = false;
       if (td_cost > probe_cost) {
         probe(hint, next, L + 2);
@@ -490,6 +493,13 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       // everything in HBM: the edge-ratio rule of Beamer et al.
       if (!bu && (double)mf > (double)mu / g.do_alpha) bu = true;
       else if (bu && (double)nf < (double)g.n / g.do_beta && nf < prev_nf) bu = false;
+      // once a probe has left candidates, a bottom-up level costs a list
+      // probe over them: taken when cheaper than the top-down compaction
+      if (!bu && cand >= 0) {
+        td_cost = td_hbm_cost(nf, mf);
+        bu_cost = list_cost();
+        bu = bu_cost < td_cost;
+      }
     }
     prev_nf = nf;
     a.next = next;
