@@ -249,24 +249,26 @@ static void check_src(Graph &g, uint32_t src) {
 }
 
 // Direction-optimising BFS on a symmetric graph (Beamer et al., SC'12).
-// Each level first counts the frontier (vertices n_f, edges m_f).  All
-// edges in HBM: with m_u the edges of still-unvisited vertices, the level
-// runs bottom-up when m_f > m_u / alpha and returns to top-down once
-// n_f < n / beta and the frontier shrinks.  Split store: what matters is
-// the host link, so when the frontier's host edges are worth it the
-// parent hints are tested first (every unvisited vertex whose hint is in
-// the frontier joins the next level: it is at level L+1 whichever way the
-// level runs), the rest of the unvisited set is counted, and the cheaper
-// direction in link time is taken (below).
+// Each level first counts the frontier (vertices n_f, edges m_f), unless
+// the last level ran bottom-up and summed it already.  All edges in HBM:
+// with m_u the edges of still-unvisited vertices, the level runs bottom-up
+// when m_f > m_u / alpha and returns to top-down once n_f < n / beta and
+// the frontier shrinks; once a probe has left candidates, a level also
+// runs bottom-up when a probe over them costs less than the top-down
+// level's compaction.  Split store: what matters is the host link, so when
+// the frontier's host edges are worth it the probe runs first (parent
+// hints: every unvisited vertex whose hint is in the frontier joins the
+// next level, at level L+1 whichever way the level runs; the rest of the
+// unvisited set is counted and sorted into short and long lists), and the
+// cheaper direction in link time is taken (below).
 //
 // A top-down level is the ordinary compaction + expansion.  A bottom-up
-// level compacts the unvisited set (inverted visited bitmap) in two parts:
-// vertices of degree <= D (do_pull_degree) are scanned in full by the
-// edge-balanced kernel (their lists are a few records, read as coalesced
-// wave loads shared by neighbouring queue entries), longer lists go to the
-// early-exit kernel, which stops at the first frontier neighbour.
-// edges_traversed keeps the algorithm-independent count (sum of degrees of
-// reached vertices); edges_examined counts the records actually read.
+// level is the probe (a sweep of every vertex the first time, the previous
+// probe's candidates after that), the full scan of the short lists
+// (degree <= D, do_pull_degree) and the early-exit kernel over the long
+// ones, which stops at the first frontier neighbour.  edges_traversed keeps
+// the algorithm-independent count (sum of degrees of reached vertices);
+// edges_examined counts the records actually read.
 static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   Workspace &w = g.ws;
   hipStream_t s = g.stream;
