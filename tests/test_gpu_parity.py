@@ -428,6 +428,14 @@ def test_single_vertex_and_isolated():
     g = T.Graph.from_arrays(off, dst)
     assert g.bfs(0)[0].tolist() == [1, 5, 5, 2]
     assert g.cc()[0].tolist() == [0, 1, 2, 0]
+    # symmetric (direction-optimising path): an isolated source ends at
+    # level 0; the values of the call before must not leak into the next
+    g.set_option("symmetric", 1)
+    assert g.bfs(0)[0].tolist() == [1, 5, 5, 2]
+    assert g.bfs(1)[0].tolist() == [5, 1, 5, 5]
+    assert g.bfs(3)[0].tolist() == [2, 5, 5, 1]
+    assert g.cc()[0].tolist() == [0, 1, 2, 0]
+    assert g.bfs(2)[0].tolist() == [5, 5, 1, 5]
 
 
 def test_hub_spanning_many_chunks():
