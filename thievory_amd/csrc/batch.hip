@@ -42,6 +42,7 @@ struct MsArgs {
   uint32_t L;
   uint32_t level;
   uint32_t voff[64];
+  unsigned long long fmask;  // BFS: lanes this expansion pushes (others run bottom-up)
 };
 
 // BFS: lane per edge, lane masks staged per queue entry.
@@ -77,7 +78,7 @@ __global__ void __launch_bounds__(MBS) k_ms_bfs(MsArgs a) {
       if (s < C) {
         atomicOr(&s_mask[s >> 5], 1u << (s & 31));
         s_base[t] = a.eptr[v] - b * REC;
-        s_m[t] = a.F[v];
+        s_m[t] = a.F[v] & a.fmask;
       }
     }
     __syncthreads();
@@ -587,6 +588,20 @@ __global__ void k_ms_admit(AdmitArgs ad, int algo, unsigned long long *F, uint32
   atomicOr(&fb[s >> 5], 1u << (s & 31));
 }
 
+// Frontier bitmap of some lanes: bit v is set when F[v] holds one of them.
+// A wave takes 64 vertices, its ballot is their two bitmap words.
+__global__ void __launch_bounds__(MBS) k_ms_lanebits(const unsigned long long *F, uint64_t n,
+                                                     unsigned long long lanes, uint32_t *bits) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (MBS / 64);
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * (MBS / 64) + (threadIdx.x >> 6)) * 64; w0 < n;
+       w0 += nw * 64) {
+    const uint64_t v = w0 + lane;
+    const unsigned long long b = __ballot(v < n && (F[v] & lanes) != 0);
+    if (lane < 2) bits[(w0 >> 5) + lane] = (uint32_t)(b >> (32 * lane));
+  }
+}
+
 __global__ void k_ms_clear(unsigned long long *seen, uint64_t n, unsigned long long keep) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * blockDim.x)
@@ -668,6 +683,8 @@ MultiEngine::MultiEngine(Graph &gr, int al, uint32_t lanes, bool kf)
   if (algo == THV_ALGO_BFS) {
     seen.alloc((n + 1) * 8);
     HIPC(hipMemsetAsync(seen.p, 0, (n + 1) * 8, stream));
+    tbits.alloc(words * 4);
+    HIPC(hipMemsetAsync(tbits.p, 0, words * 4, stream));
     if (keep_full) full.alloc((uint64_t)L * n * 4 + 4);
   } else {
     dist.alloc((uint64_t)L * n * 4 + 4);
@@ -695,7 +712,8 @@ uint64_t MultiEngine::bytes_needed(const Graph &g, int algo, uint32_t lanes, boo
   uint64_t b = 2 * (words * 4 + (n + 1) * 8);                       // fb, F
   b += 2 * ((n + 1) * 12) + 2 * (g.m / 1024 + 2) * 4;                // queues
   b += (n / TILE + 1) * TILE * 4;                                     // long-list queue
-  if (algo == THV_ALGO_BFS) b += (n + 1) * 8 + (keep_full ? (uint64_t)lanes * n * 4 : 0);
+  if (algo == THV_ALGO_BFS)
+    b += (n + 1) * 8 + words * 4 + (keep_full ? (uint64_t)lanes * n * 4 : 0);
   else b += (uint64_t)lanes * n * 4 + (keep_full ? n * 4 : 0);
   return b;
 }
@@ -752,6 +770,8 @@ void MultiEngine::launch_level() {
       ad.count++;
       voff[q] = level;
       last_val[q] = 1;
+      fsz[q] = 1;       // the source
+      cnt_prev[q] = 1;
     }
     hipLaunchKernelGGL(k_ms_admit, dim3(1), dim3(64), 0, stream, ad, algo,
                        F[cur].as<unsigned long long>(), fb[cur].as<uint32_t>(),
@@ -776,8 +796,23 @@ void MultiEngine::launch_level() {
       HIPC(hipStreamSynchronize(stream));
       const uint64_t mf = h_ctr[level & 1].qw[0] + h_ctr[level & 1].qw[1];
       level_bu = BU_ALPHA * mf > mu_est && mf > n / 8;
+      // lanes with small frontiers run this level top-down
+      uint64_t td = 0;
       if (level_bu) {
-        launch_bottom_up();
+        const uint64_t thr = std::max<uint64_t>(1, n / LANE_TD_DIV);
+        for (uint32_t q = 0; q < MAXL; q++)
+          if (((active >> q) & 1ull) && fsz[q] < thr) td |= 1ull << q;
+        if (td == active) level_bu = false;  // every lane's frontier is small
+      }
+      level_td_lanes = level_bu ? td : 0;
+      if (level_bu) {
+        launch_bottom_up(active & ~td);
+        if (td) {
+          hipLaunchKernelGGL(k_ms_lanebits, dim3(grid_for(n, 8 * g.cus)), dim3(MBS), 0, stream,
+                             F[cur].as<unsigned long long>(), n, td, tbits.as<uint32_t>());
+          launch_compact(g, qs, tbits.as<uint32_t>(), c, stream);
+          launch_top_down(c, td);
+        }
       } else {
         launch_compact(g, qs, fb[cur].as<uint32_t>(), c, stream, CMP_WRITE_ONLY);
         launch_top_down(c);
@@ -791,7 +826,7 @@ void MultiEngine::launch_level() {
   HIPC(hipMemcpyAsync(h_acc, acc.p, ACC_WORDS * 8, hipMemcpyDeviceToHost, stream));
 }
 
-void MultiEngine::launch_bottom_up() {
+void MultiEngine::launch_bottom_up(uint64_t lanes) {
   const uint64_t n = g.n;
   HIPC(hipMemsetAsync(acc.as<unsigned long long>() + ACC_LONG, 0, 4 * 8, stream));
   MsBuArgs b;
@@ -808,14 +843,14 @@ void MultiEngine::launch_bottom_up() {
   b.acc = acc.as<unsigned long long>();
   b.lq = qs.slist.as<uint32_t>();
   b.n = n;
-  b.act = active;
+  b.act = lanes;
   b.level = level;
   for (uint32_t q = 0; q < MAXL; q++) b.voff[q] = voff[q];
   // one value for all active lanes (a cohort admitted together): hash each
   // gained vertex once
   b.uval = 0;
   for (uint32_t q = 0; q < MAXL; q++) {
-    if (!((active >> q) & 1ull)) continue;
+    if (!((lanes >> q) & 1ull)) continue;
     const uint32_t v = level + 2 - voff[q];
     if (!b.uval) b.uval = v;
     else if (b.uval != v) {
@@ -835,7 +870,7 @@ void MultiEngine::launch_bottom_up() {
   HIPC(hipGetLastError());
 }
 
-void MultiEngine::launch_top_down(LevelCtr *c) {
+void MultiEngine::launch_top_down(LevelCtr *c, uint64_t lanes) {
   const uint64_t n = g.n;
   {
     MsArgs a;
@@ -857,6 +892,7 @@ void MultiEngine::launch_top_down(LevelCtr *c) {
     a.n = n;
     a.L = L;
     a.level = level;
+    a.fmask = lanes;
     for (uint32_t q = 0; q < MAXL; q++) a.voff[q] = voff[q];
     const int grid = g.shape.blocks_per_cu * g.cus;
     const bool r8 = g.rec == 8;
@@ -913,6 +949,14 @@ void MultiEngine::finish_level(std::vector<LaneResult> &done) {
     stats.hbm_bytes += sd * (uint64_t)g.rec;
     stats.bottom_up_levels++;
     mu_est = h_acc[ACC_MU];
+    if (level_td_lanes) {  // the lanes that ran top-down in this level
+      const LevelCtr &h = h_ctr[level & 1];
+      stats.edges_traversed += h.qw[0] + h.qw[1];
+      stats.host_bytes += h.qw[0] * (uint64_t)g.rec;
+      stats.hbm_bytes += h.qw[1] * (uint64_t)g.rec;
+      // their missing vertices' edges are not in the bottom-up measure
+      mu_est = g.m;
+    }
   } else {
     const LevelCtr &h = h_ctr[level & 1];
     stats.edges_traversed += h.qw[0] + h.qw[1];
@@ -922,6 +966,11 @@ void MultiEngine::finish_level(std::vector<LaneResult> &done) {
   stats.iterations++;
   stats.expand_launches++;
   const uint64_t lm = h_acc[ACC_LM];
+  for (uint32_t q = 0; q < MAXL; q++) {  // each lane's next frontier: this level's gains
+    if (!((active >> q) & 1ull)) continue;
+    fsz[q] = h_acc[ACC_CNT + q] - cnt_prev[q];
+    cnt_prev[q] = h_acc[ACC_CNT + q];
+  }
   for (uint32_t q = 0; q < L; q++) {
     const uint64_t bit = 1ull << q;
     if (!(active & bit)) continue;
