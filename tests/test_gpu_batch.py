@@ -76,6 +76,45 @@ def test_batch_bfs_bottom_up_levels(weights, frac):
     assert bu["stats"].edges_traversed < td["stats"].edges_traversed
 
 
+def test_batch_split_direction_lanes():
+    """A bottom-up batch level splits its lanes: lanes whose frontier is a
+    few vertices (sources on a long path hanging off an R-MAT core) run
+    top-down while the core's lanes run bottom-up; every lane equals the
+    oracle, across the levels where the path lanes enter the core."""
+    _say("case: batch split lanes")
+    core = O.rmat(16, 16, seed=41)
+    n0, plen = core.n, 600
+    n = n0 + plen
+    deg0 = np.diff(core.off)
+    hub = int(np.argmax(deg0))
+    src = np.repeat(np.arange(n0), deg0.astype(np.int64))
+    u = np.concatenate([src, np.arange(n0, n - 1), np.arange(n0 + 1, n), [hub, n0]])
+    v = np.concatenate([core.adj.astype(np.int64), np.arange(n0 + 1, n), np.arange(n0, n - 1),
+                        [n0, hub]])
+    p = np.stack([u, v], 1)
+    p = p[np.lexsort((p[:, 1], p[:, 0]))]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, p[:, 0] + 1, 1)
+    og = O.CSR(np.cumsum(off).astype(np.uint64), p[:, 1].astype(np.uint32))
+    g = T.Graph.from_arrays(og.off, og.adj)
+    g.set_option("symmetric", 1)
+    rng = np.random.default_rng(17)
+    srcs = np.concatenate([rng.integers(0, n0, 32), n - 1 - np.arange(32) * 7]).astype(np.uint32)
+    out = g.batch("bfs", srcs, full=True)
+    assert out["stats"].bottom_up_levels > 0
+    for i, s in enumerate(srcs):
+        assert np.array_equal(out["full"][i], O.bfs(og, int(s))), (i, s)
+    # the same lanes served with staggered admissions
+    srv = T.Server(g, max_lanes=64, keep_full=True)
+    tk = [(int(s), srv.submit("bfs", int(s))) for s in srcs[::-1]]
+    srv.drain()
+    for s, t in tk:
+        full = np.zeros(og.n, np.uint32)
+        r = srv.poll(t, full)
+        assert r["status"] == 0 and np.array_equal(full, O.bfs(og, s)), s
+    srv.stop()
+
+
 @pytest.mark.parametrize("n", [5, 63, 1000])
 def test_batch_bottom_up_small_graphs(n):
     """Bottom-up batch levels on small symmetric graphs whose vertex count is
