@@ -744,7 +744,9 @@ void MultiEngine::launch_level() {
   const uint64_t words = bitmap_words(n);
   launched_expand = false;
   launched_finish = false;
-  if (pend_clear) {
+  // finished BFS lanes keep their seen bits until a query is admitted (no
+  // kernel reads the bits of an inactive lane); one pass clears them all
+  if (pend_clear && !admit_q.empty()) {
     hipLaunchKernelGGL(k_ms_clear, dim3(grid_for(n, 8192)), dim3(MBS), 0, stream,
                        seen.as<unsigned long long>(), n, ~pend_clear);
     pend_clear = 0;

@@ -52,7 +52,7 @@ struct MultiEngine {
   // host lane state
   uint64_t active = 0;         // lanes traversing
   uint64_t pend_finish = 0;    // SSSP lanes done, digest pending
-  uint64_t pend_clear = 0;     // BFS lanes done, seen bits to clear
+  uint64_t pend_clear = 0;     // BFS lanes done, seen bits to clear before an admission
   uint64_t finalizing = 0;     // lanes whose digest is in flight this level
   uint32_t voff[MAXL] = {};
   uint32_t last_val[MAXL] = {};
@@ -94,7 +94,7 @@ struct MultiEngine {
   uint64_t free_mask() const;
   uint32_t free_lanes() const { return (uint32_t)__builtin_popcountll(free_mask()); }
   int admit(uint32_t src, uint64_t tag);   // -> lane, or -1 when none is free
-  bool busy() const { return active || pend_finish || pend_clear || !admit_q.empty(); }
+  bool busy() const { return active || pend_finish || !admit_q.empty(); }
   void launch_level();                     // enqueue one level on `stream`
   // expansion of the compacted frontier, pushing the given lanes
   void launch_top_down(LevelCtr *c, uint64_t lanes = ~0ull);
