@@ -1,15 +1,34 @@
-"""Benchmark: BFS GTEPS on R-MAT scale 26 / edge factor 32 with the device
-edge budget capped at 25% of the edge bytes (the rest read zero-copy from
-pinned host memory), partitions of 2^16 edges; plus the CC pass of the same
-configuration and the multithreaded CPU baseline.
+"""Benchmark of the out-of-memory frontier engine on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 26] ...
+Headline (`value`): BFS GTEPS on R-MAT scale 26 / edge factor 32 with the
+device edge budget capped at 25% of the edge bytes (the rest read zero-copy
+from pinned host memory), partitions of 2^16 edges, from vertex 1.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] ...
 
 One step = one BFS from vertex 1 over the whole graph (first launch to
 empty frontier; graph generation and store set-up are outside the timed
 region).  Each engine call returns only after its HIP stream has drained, so
-the host clock around the K steps brackets all device work.  Prints ONE
-JSON line.
+the host clock around the K steps brackets all device work.
+
+The same run also measures, each outside the timed region of the headline
+and each checked against the CPU oracle at full size:
+  multi_root   64 seeded random roots of the same graph (harmonic-mean
+               GTEPS), every root's levels against the oracle's multi-source
+               BFS
+  cc           connected components of the same graph, labels against the
+               oracle's label propagation
+  c2_topdown   the store as the configuration states it (densest partitions
+               in HBM) with the reference's top-down traversal: link rate and
+               mixed roofline
+  sssp         SSSP on R-MAT 26 / ef 16, 8-byte records, 30% budget
+  pr           PageRank push and pull on directed R-MAT 26 / ef 16, 25%
+               budget, normalised L1 against the FP64 fixpoint
+  serving      resident R-MAT 25 / ef 16 with weights: one query per launch,
+               64-lane batches (every lane against the oracle), and the
+               continuous-batching server on Poisson streams of 20k queries
+               (256 checked queries spread through the mixed stream)
+Prints ONE JSON line.
 """
 import argparse
 import json
@@ -32,17 +51,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def oracle():
+    """The CPU checker (oracle/): the cpu_baseline leg and the parity checks
+    use it; nothing measured goes through it."""
+    from oracle import oracle as O
+    return O
+
+
 def mixed_roofline_ms(hbm_bytes, host_bytes):
     return max(hbm_bytes / HBM_PEAK, host_bytes / LINK_PEAK) * 1e3
 
 
-def cpu_baseline(g, src, seconds, label):
+def cpu_meta(O):
+    return {"cores": O.threads(), "omp_threads": O.threads(),
+            "hardware_concurrency": os.cpu_count()}
+
+
+def cpu_baseline(O, og, src, seconds, label):
     """BFS on the host cores with the oracle's multithreaded restatement, on
     the same graph, bounded to roughly `seconds` of work."""
-    from oracle import oracle as O
-    off, dst, _, _ = g.export()
-    og = O.CSR(off, dst)
     runs, total, trav = 0, 0.0, 0
+    lv = None
     while total < seconds or runs == 0:
         t0 = time.perf_counter()
         lv, tr = O.bfs_mt(og, src)
@@ -51,10 +80,11 @@ def cpu_baseline(g, src, seconds, label):
         runs += 1
         if runs >= 20:
             break
-    return {"value": trav / total / 1e9, "unit": "GTEPS", "cores": O.threads(),
-            "kind": "port",
-            "sample": f"{runs} full BFS run(s) from vertex {src} on the same {label} graph "
-                      f"(oracle/oracle.c orc_bfs_mt, OpenMP top-down), {total:.1f} s"}, lv
+    out = {"value": trav / total / 1e9, "unit": "GTEPS", "kind": "port",
+           "sample": f"{runs} full BFS run(s) from vertex {src} on the same {label} graph "
+                     f"(oracle/oracle.c orc_bfs_mt, OpenMP top-down), {total:.1f} s"}
+    out.update(cpu_meta(O))
+    return out, lv
 
 
 def pct(xs, p):
@@ -62,108 +92,17 @@ def pct(xs, p):
     return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))] if xs else None
 
 
-def serve_stream(T, g, queries, rate, seed):
-    """Submits (algo, src) queries to a fresh server, at Poisson arrivals of
-    `rate` per second (None: all at once); returns throughput and latency."""
-    srv = T.Server(g, max_lanes=64)
-    rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, len(queries)) if rate else np.zeros(len(queries))
-    t0 = time.perf_counter()
-    due = 0.0
-    tickets = []
-    for (a, s), gap in zip(queries, gaps):
-        due += gap
-        while rate:
-            left = due - (time.perf_counter() - t0)
-            if left <= 0:
-                break
-            time.sleep(min(0.0005, left))
-        tickets.append((a, srv.submit(a, s)))
-    srv.drain()
-    wall = time.perf_counter() - t0
-    res = [(a, srv.poll(t)) for a, t in tickets]
-    srv.stop()
-    assert all(r["status"] == 0 for _, r in res)
-    first = min(r["submit_ms"] for _, r in res)
-    last = max(r["done_ms"] for _, r in res)
-    out = {"queries": len(res), "qps": round(len(res) / ((last - first) / 1e3), 2),
-           "wall_s": round(wall, 3)}
-    for name in ("all", "bfs", "sssp"):
-        lat = [r["done_ms"] - r["submit_ms"] for a, r in res if name in ("all", a)]
-        if lat:
-            out[f"{name}_p50_ms"] = round(pct(lat, 50), 3)
-            out[f"{name}_p99_ms"] = round(pct(lat, 99), 3)
-    return out, res
-
-
-def serving_bench(T, args):
-    """Resident R-MAT scale 25 / ef 16 with weights in [1, 64]: one query per
-    launch vs. 64-lane batches vs. the continuous-batching server."""
-    t0 = time.perf_counter()
-    g = T.Graph.rmat(args.serve_scale, 16, seed=2, symmetric=True, weights=True)
-    g.store_config(None, 1 << 16)
-    log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f} s)")
-    rng = np.random.default_rng(3)
-    out = {"graph": {"scale": args.serve_scale, "edge_factor": 16, "vertices": g.n,
-                     "edges": g.m, "edge_bytes": 8, "resident": True}}
-    # the same 64 uniform random sources: one query per launch sequence
-    # (plain single-source kernels) vs. one 64-lane batch
-    b64 = rng.integers(0, g.n, 64).astype(np.uint32)
-    deg = np.diff(g.offsets()).astype(np.int64)
-    out["sources_isolated"] = int((deg[b64] == 0).sum())
-    def plain_qps(fn):
-        fn(int(b64[0]), out=False)
-        t = time.perf_counter()
-        for s in b64:
-            fn(int(s), out=False)
-        return round(64 / (time.perf_counter() - t), 2)
-
-    # plain BFS two ways: the reference's one-query-per-launch top-down
-    # kernel (the serving target's baseline), and this engine's own
-    # direction-optimising single query
-    g.set_option("direction", 0)
-    out["plain_bfs_topdown_qps"] = plain_qps(g.bfs)
-    g.set_option("direction", 1)
-    out["plain_bfs_qps"] = plain_qps(g.bfs)
-    out["plain_sssp_qps"] = plain_qps(g.sssp)
-    for algo in ("bfs", "sssp"):
-        r = g.batch(algo, b64)
-        out[f"batch64_{algo}_qps"] = round(64 / (r["stats"].ms / 1e3), 2)
-        out[f"batch64_{algo}_levels"] = r["stats"].iterations
-        out[f"batch64_{algo}_vs_plain"] = round(out[f"batch64_{algo}_qps"] /
-                                                out[f"plain_{algo}_qps"], 2)
-    out["batch64_bfs_vs_plain_topdown"] = round(out["batch64_bfs_qps"] /
-                                                out["plain_bfs_topdown_qps"], 2)
-    log(f"plain/batch: {out}")
-    # BFS-only stream: saturation, then Poisson at 70% of it
-    nb = args.serve_queries // 2
-    qb = [("bfs", int(s)) for s in rng.integers(0, g.n, nb)]
-    sat, _ = serve_stream(T, g, qb, None, 4)
-    pois, _ = serve_stream(T, g, qb, 0.7 * sat["qps"], 5)
-    out["serve_bfs_saturated"] = sat
-    out["serve_bfs_poisson70"] = pois
-    out["serve_bfs_speedup_vs_plain"] = round(sat["qps"] / out["plain_bfs_qps"], 2)
-    out["serve_bfs_speedup_vs_plain_topdown"] = round(sat["qps"] /
-                                                      out["plain_bfs_topdown_qps"], 2)
-    log(f"bfs stream: {sat} / {pois}")
-    # the mixed 50/50 BFS+SSSP stream of the configuration
-    qm = [("bfs" if rng.random() < 0.5 else "sssp", int(s))
-          for s in rng.integers(0, g.n, args.serve_queries)]
-    msat, _ = serve_stream(T, g, qm, None, 6)
-    mpois, _ = serve_stream(T, g, qm, 0.7 * msat["qps"], 7)
-    out["serve_mix_saturated"] = msat
-    out["serve_mix_poisson70"] = mpois
-    log(f"mix stream: {msat} / {mpois}")
-    g.close()
-    return out
+def hmean(xs):
+    xs = [x for x in xs if x > 0]
+    return len(xs) / sum(1.0 / x for x in xs) if xs else 0.0
 
 
 def measured_traffic(workload, kernel):
     """Memory traffic per launch of `kernel` in `workload` from the committed
     counter summary profiles/traffic.json: rocprofv3 FETCH_SIZE and
     WRITE_SIZE per dispatch (KiB), each divided by what the counter reads
-    for a plain copy of known size (thievory_amd/calibrate.py).  None when
-    no such measurement is on file."""
+    for a plain copy of known size.  None when no such measurement is on
+    file."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             e = json.load(f).get(workload, {}).get(kernel)
@@ -172,18 +111,19 @@ def measured_traffic(workload, kernel):
     return int(e["bytes_per_launch"]) if e else None
 
 
+# ---------------------------------------------------------------- C2 BFS ---
+
 def bfs_roofline(stats, workload=None):
     """Roofline of the BFS steps.  Kernel view: the traversal kernels'
     algorithmic bytes over their HIP-event time -- the bottom-up probe
     (the first level's sweep or a later level's candidate-list probe, then
-    the short-list scan: one event window) against HBM (its reads are the
-    visited / residency / hint / degree words, candidates, short lists and
-    frontier words, all in HBM), the top-down expansion and the early-exit
-    bottom-up window against the host link.
-    The dominant kernel (most time) is the headline.  Mixed views: T_roof =
-    max(HBM bytes / 8 TB/s, host bytes / 64 GB/s) against the step time,
-    once with the bytes this run read and once with the bytes a
-    top-down-only traversal of the same levels reads."""
+    the short-list scan: one event window) against HBM, the top-down
+    expansion and the early-exit bottom-up window against the host link.
+    The dominant kernel (most time) is the headline.  Mixed view: T_roof =
+    max(HBM bytes / 8 TB/s, host bytes / 64 GB/s) of the bytes this run read,
+    against the step time.  The top-down view prices the bytes a
+    top-down-only traversal of the same levels would read: a time ratio, not
+    a fraction of a roofline (this run does not read those bytes)."""
     med = statistics.median(s.ms for s in stats)
     kern = {}
     for name, ms_key, launches, nbytes, bound in (
@@ -218,23 +158,176 @@ def bfs_roofline(stats, workload=None):
         "mixed": {"t_roof_ms": round(t_act, 3), "t_ms": round(med, 3),
                   "frac": round(t_act / med, 4), "bytes": "read by this run",
                   "host_bytes": stats[0].host_bytes, "hbm_bytes": stats[0].hbm_bytes},
-        "mixed_topdown": {"t_roof_ms": round(t_td, 3), "t_ms": round(med, 3),
-                          "frac": round(t_td / med, 4),
-                          "bytes": "top-down algorithmic (every reached vertex's edges)",
-                          "host_bytes": stats[0].topdown_host_bytes,
-                          "hbm_bytes": stats[0].topdown_hbm_bytes},
+        "topdown_equivalent": {
+            "t_roof_ms": round(t_td, 3), "t_ms": round(med, 3),
+            "time_ratio": round(t_td / med, 4),
+            "bytes": "what a top-down-only traversal of the same levels would read "
+                     "(every reached vertex's edges); this run does not read them",
+            "host_bytes": stats[0].topdown_host_bytes,
+            "hbm_bytes": stats[0].topdown_hbm_bytes},
         "bottom_up_levels": stats[0].bottom_up_levels,
         "edges_examined": stats[0].edges_examined,
         "hint_hits": stats[0].hint_hits,
     }
 
 
+def multi_root(g, og, O, args):
+    """BFS from 64 seeded random roots of degree > 0 (one traversal each);
+    harmonic-mean GTEPS, and every root's levels checked against the
+    oracle's 64-query BFS (digest, reached count, depth)."""
+    deg = np.diff(og.off)
+    rng = np.random.default_rng(11)
+    roots = rng.choice(np.flatnonzero(deg > 0), args.roots, replace=False).astype(np.uint32)
+    rates, ms, digests = [], [], []
+    for r in roots:
+        lv, st = g.bfs(int(r))
+        rates.append(st.edges_traversed / st.ms / 1e6)
+        ms.append(st.ms)
+        digests.append(O.hash_values(lv, g.n + 1))
+    out = {"roots": len(roots), "seed": 11, "hmean_gteps": round(hmean(rates), 3),
+           "min_gteps": round(min(rates), 3), "max_gteps": round(max(rates), 3),
+           "median_ms": round(statistics.median(ms), 3)}
+    if not args.no_check:
+        t = time.perf_counter()
+        ok = 0
+        for i in range(0, len(roots), 64):
+            ref = O.bfs_multi(og, roots[i:i + 64], symmetric=True)
+            for j in range(len(ref["hash"])):
+                h, rc, d = digests[i + j]
+                ok += (int(ref["hash"][j]), int(ref["reached"][j]), int(ref["depth"][j])) == \
+                    (int(h), int(rc), int(d))
+        out["oracle_checked"] = len(roots)
+        out["oracle_matches"] = ok
+        out["all_match"] = ok == len(roots)
+        out["check_s"] = round(time.perf_counter() - t, 1)
+    return out
+
+
+def c2_topdown(g, cpu_lv, args):
+    """The store the configuration states (densest partitions static in HBM,
+    the rest zero-copy) with the reference's top-down traversal: no parent
+    hints, no bottom-up levels."""
+    g.set_option("parent_hints", 0)   # rebuilds the store dense-first, same budget
+    g.set_option("direction", 0)
+    store = g.store_info()
+    g.bfs(args.source, out=False)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.topdown_steps):
+        stats.append(g.bfs(args.source, out=False)[1])
+    wall = time.perf_counter() - t0
+    lv, _ = g.bfs(args.source)
+    s0 = stats[0]
+    exp_ms = sum(s.expand_ms for s in stats)
+    link = sum(s.host_bytes for s in stats) / (exp_ms * 1e-3) / 1e9
+    med = statistics.median(s.ms for s in stats)
+    t_roof = statistics.median(mixed_roofline_ms(s.hbm_bytes, s.host_bytes) for s in stats)
+    out = {"store": store, "layout": "dense-first (highest average degree in HBM)",
+           "steps": args.topdown_steps, "ms_median": round(med, 3),
+           "gteps": round(sum(s.edges_traversed for s in stats) / wall / 1e9, 3),
+           "levels": s0.iterations, "host_bytes": s0.host_bytes, "hbm_bytes": s0.hbm_bytes,
+           "host_partitions_touched": s0.host_partitions_touched,
+           "host_partitions_skipped": s0.host_partitions_skipped,
+           "kernel": {"name": "k_expand<BFS>", "bound": "host-link",
+                      "achieved_GBps": round(link, 2), "peak_GBps": LINK_PEAK / 1e9,
+                      "frac": round(link * 1e9 / LINK_PEAK, 4),
+                      "avg_launch_ms": round(exp_ms / sum(s.expand_launches for s in stats), 4)},
+           "mixed": {"t_roof_ms": round(t_roof, 3), "t_ms": round(med, 3),
+                     "frac": round(t_roof / med, 4)}}
+    if cpu_lv is not None:
+        out["cpu_matches_gpu"] = bool(np.array_equal(cpu_lv, lv))
+    return out
+
+
+def c2_block(T, O, args):
+    t0 = time.perf_counter()
+    g = T.Graph.rmat(args.scale, args.edge_factor, seed=1, symmetric=True)
+    log(f"generated R-MAT scale {args.scale} ef {args.edge_factor}: n={g.n} m={g.m} "
+        f"in {time.perf_counter() - t0:.1f} s")
+    budget = int(args.budget * g.m * g.edge_bytes)
+    t0 = time.perf_counter()
+    g.store_config(budget, args.partition_edges)
+    store = g.store_info()
+    store["hint_bytes"] = int(g.get_option("hint_bytes"))
+    log(f"store: {store} (budget {budget} B) in {time.perf_counter() - t0:.1f} s")
+
+    for i in range(args.warmup):
+        _, st = g.bfs(args.source, out=False)
+        log(f"warmup {i}: {st.ms:.1f} ms, {st.iterations} levels")
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = g.bfs(args.source, out=False)
+        stats.append(st)
+    wall = time.perf_counter() - t0
+    for i, st in enumerate(stats):
+        log(f"step {i}: {st.ms:.2f} ms, {st.edges_traversed / st.ms / 1e6:.2f} GTEPS, "
+            f"probe {st.probe_ms:.2f} ms, expand {st.expand_ms:.2f} ms, "
+            f"bottom-up {st.bottom_up_ms:.2f} ms, "
+            f"levels {st.iterations} ({st.bottom_up_levels} bottom-up)")
+    trav = sum(s.edges_traversed for s in stats)
+    value = trav / wall / 1e9
+    workload = f"bfs-rmat{args.scale}-ef{args.edge_factor}-budget{int(args.budget * 100)}pct"
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded R-MAT, generated on the GPU)",
+        "config": {"workload": workload,
+                   "scale": args.scale, "edge_factor": args.edge_factor, "vertices": g.n,
+                   "edges": g.m, "edge_bytes": g.edge_bytes, "budget_frac": args.budget,
+                   "partition_edges": args.partition_edges, "source": args.source,
+                   "store": store, "graph500_gteps": round(value / 2, 4)},
+        "roofline": bfs_roofline(stats, workload),
+    }
+    lv, _ = g.bfs(args.source)  # levels for the CPU cross-check (untimed)
+    og = None
+    if not (args.no_cpu and args.no_check):
+        off, dst, _, _ = g.export()
+        og = O.CSR(off, dst)
+    if not args.no_cc:
+        cc, cst = g.cc()
+        out["cc"] = {"ms": round(cst.ms, 2), "edges_processed": cst.edges_traversed,
+                     "iterations": cst.iterations,
+                     "graph_edges_per_s_g": round(g.m / cst.ms / 1e6, 3),
+                     "method": "union-find" if g.get_option("cc_union") else "label propagation",
+                     "host_bytes": cst.host_bytes,
+                     "mixed_roofline_frac": round(
+                         mixed_roofline_ms(cst.hbm_bytes, cst.host_bytes) / cst.ms, 4)}
+        if not args.no_check:
+            t = time.perf_counter()
+            ref = O.cc_mt(og)
+            out["cc"]["oracle_matches"] = bool(np.array_equal(ref, cc))
+            out["cc"]["components"] = int(np.count_nonzero(ref == np.arange(g.n)))
+            out["cc"]["check_s"] = round(time.perf_counter() - t, 1)
+            out["cc"]["oracle"] = "orc_cc_mt (label propagation)"
+        del cc
+        log(f"cc: {out['cc']}")
+    cpu_lv = None
+    if not args.no_cpu:
+        cb, cpu_lv = cpu_baseline(O, og, args.source, args.cpu_seconds,
+                                  f"R-MAT {args.scale}/{args.edge_factor}")
+        out["cpu_baseline"] = cb
+        out["cpu_matches_gpu"] = bool(np.array_equal(cpu_lv, lv))
+        log(f"cpu baseline: {cb}")
+    if args.roots:
+        out["multi_root"] = multi_root(g, og, O, args)
+        log(f"multi-root: {out['multi_root']}")
+    if not args.no_topdown:
+        out["c2_topdown"] = c2_topdown(g, cpu_lv, args)
+        log(f"c2 top-down: {out['c2_topdown']}")
+    g.close()
+    return out
+
+
+# --------------------------------------------------------- C3 / C4 blocks ---
+
 def step_summary(stats, wall, kernel):
     """Throughput, mixed roofline and expansion-kernel link figures of K runs."""
     exp_ms = sum(s.expand_ms for s in stats)
     launches = sum(s.expand_launches for s in stats)
     host_all = sum(s.host_bytes for s in stats)
-    hbm_all = sum(s.hbm_bytes for s in stats)
     med = statistics.median(s.ms for s in stats)
     t_roof = statistics.median(mixed_roofline_ms(s.hbm_bytes, s.host_bytes) for s in stats)
     link = host_all / (exp_ms * 1e-3) / 1e9 if exp_ms else 0.0
@@ -252,10 +345,12 @@ def step_summary(stats, wall, kernel):
                                "frac": round(t_roof / med, 4)}},
         "host_bytes_per_run": stats[0].host_bytes,
         "hbm_bytes_per_run": stats[0].hbm_bytes,
+        "host_partitions_touched": stats[0].host_partitions_touched,
+        "host_partitions_skipped": stats[0].host_partitions_skipped,
     }
 
 
-def sssp_bench(T, args):
+def sssp_block(T, O, args):
     """SSSP from a seeded random source with degree > 0 on R-MAT scale 26 /
     ef 16, 8-byte {dst, weight} records, weights in [1, 64], 30% HBM budget."""
     g = T.Graph.rmat(args.scale, 16, seed=1, symmetric=True, weights=True)
@@ -266,41 +361,42 @@ def sssp_bench(T, args):
     g.sssp(src, out=False)
     stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(args.sssp_steps):
         stats.append(g.sssp(src, out=False)[1])
     out = {"workload": f"sssp-rmat{args.scale}-ef16-w8-budget30pct", "source": src,
            "vertices": g.n, "edges": g.m, "store": g.store_info()}
     out.update(step_summary(stats, time.perf_counter() - t0, "k_expand<SSSP>"))
     dist, _ = g.sssp(src)
     # algorithm-independent rate: edges out of the reached vertices per
-    # second (delta-stepping relaxes fewer edges than Bellman-Ford, so the
-    # processed-edge rate alone would understate a faster run)
+    # second (delta-stepping relaxes fewer edges than Bellman-Ford)
     reached_edges = int(deg[dist != g.n + 1].sum())
     out["reached_edges"] = reached_edges
     out["reached_edge_gteps"] = round(reached_edges / out["ms_median"] / 1e6, 4)
     out["sssp_delta"] = g.get_option("sssp_delta")
     if not args.no_cpu:
-        from oracle import oracle as O
         off, dst, w, _ = g.export()
+        og = O.CSR(off, dst, w)
         t = time.perf_counter()
-        cd = O.sssp_mt(O.CSR(off, dst, w), src)
+        cd = O.sssp_mt(og, src)
         ct = time.perf_counter() - t
-        out["cpu_baseline"] = {"value": round(stats[0].edges_traversed / ct / 1e9, 4),
-                               "unit": "G processed edges/s", "cores": O.threads(),
-                               "kind": "port", "seconds": round(ct, 2),
-                               "sample": "1 frontier Bellman-Ford run (orc_sssp_mt)"}
+        cb = {"value": round(stats[0].edges_traversed / ct / 1e9, 4),
+              "unit": "G processed edges/s", "kind": "port", "seconds": round(ct, 2),
+              "sample": "1 frontier Bellman-Ford run (orc_sssp_mt) from the same source"}
+        cb.update(cpu_meta(O))
+        out["cpu_baseline"] = cb
         out["cpu_matches_gpu"] = bool(np.array_equal(cd, dist))
     g.close()
     return out
 
 
-def pr_bench(T, args):
+def pr_block(T, O, args):
     """PageRank push (CSR) and pull (CSC) on directed R-MAT scale 26 / ef 16,
     25% HBM budget, fp32, stop at max |delta| < 1e-7 or 100 iterations;
     normalised L1 against the FP64 fixpoint."""
     out = {"workload": f"pr-rmat{args.scale}-ef16-directed-budget25pct"}
     res = {}
-    for mode, transpose in (("push", False), ("pull", True)):
+    ex = None
+    for mode, transpose in (("pull", True), ("push", False)):
         g = T.Graph.rmat(args.scale, 16, seed=1, symmetric=False, transpose=transpose)
         g.store_config(int(0.25 * g.m * 4), args.partition_edges)
         t0 = time.perf_counter()
@@ -310,39 +406,235 @@ def pr_bench(T, args):
         s["vertices"], s["edges"] = g.n, g.m
         out[mode] = s
         res[mode] = r
-        if transpose and not args.no_cpu:
-            from oracle import oracle as O
+        if transpose and not args.no_check:
             off, src, _, od = g.export()
             t = time.perf_counter()
             ex, it = O.pr_exact_mt(O.CSR(off, src, None, od), tol=1e-10, max_iter=300)
-            out["fp64_oracle"] = {"iterations": it, "seconds": round(time.perf_counter() - t, 1),
-                                  "cores": O.threads()}
-            for k in res:
-                out[k]["l1_vs_fp64"] = O.normalized_l1(res[k], ex)
+            fp = {"iterations": it, "seconds": round(time.perf_counter() - t, 1)}
+            fp.update(cpu_meta(O))
+            out["fp64_oracle"] = fp
         g.close()
+        log(f"pr {mode}: {s['ms_median']} ms")
+    if ex is not None:
+        for k in res:
+            out[k]["l1_vs_fp64"] = O.normalized_l1(res[k], ex)
+            out[k]["within_1e-5"] = out[k]["l1_vs_fp64"] <= 1e-5
     return out
+
+
+# ------------------------------------------------------------ C5 serving ---
+
+def serve_stream(T, g, queries, rate, seed):
+    """Submits (algo, src) queries to a fresh server, at Poisson arrivals of
+    `rate` per second (None: all at once); returns throughput, latency and
+    the per-query results."""
+    srv = T.Server(g, max_lanes=64)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(queries)) if rate else np.zeros(len(queries))
+    t0 = time.perf_counter()
+    due = 0.0
+    tickets = []
+    for (a, s), gap in zip(queries, gaps):
+        due += gap
+        while rate:
+            left = due - (time.perf_counter() - t0)
+            if left <= 0:
+                break
+            time.sleep(min(0.0005, left))
+        tickets.append((a, srv.submit(a, s)))
+    srv.drain()
+    wall = time.perf_counter() - t0
+    res = [(a, srv.poll(t)) for a, t in tickets]
+    srv.stop()
+    assert all(r["status"] == 0 for _, r in res)
+    first = min(r["submit_ms"] for _, r in res)
+    last = max(r["done_ms"] for _, r in res)
+    out = {"queries": len(res), "qps": round(len(res) / ((last - first) / 1e3), 2),
+           "wall_s": round(wall, 3), "offered_qps": round(rate, 1) if rate else None}
+    for name in ("all", "bfs", "sssp"):
+        lat = [r["done_ms"] - r["submit_ms"] for a, r in res if name in ("all", a)]
+        if lat:
+            out[f"{name}_p50_ms"] = round(pct(lat, 50), 3)
+            out[f"{name}_p99_ms"] = round(pct(lat, 99), 3)
+    return out, [r for _, r in res]
+
+
+def check_served(results, queries, checked, expect):
+    """Compares the served results at the positions `checked` with the
+    oracle's digest for the same (algo, source)."""
+    ok = 0
+    for i in checked:
+        a, s = queries[i]
+        r = results[i]
+        ok += (r["hash"], r["reached"], r["depth"]) == expect[(a, s)]
+    return {"checked": len(checked), "matches": ok, "all_match": ok == len(checked)}
+
+
+def serving_block(T, O, args):
+    """Resident R-MAT scale 25 / ef 16 with weights in [1, 64]: one query per
+    launch vs. 64-lane batches vs. the continuous-batching server."""
+    t0 = time.perf_counter()
+    g = T.Graph.rmat(args.serve_scale, 16, seed=2, symmetric=True, weights=True)
+    g.store_config(None, 1 << 16)
+    log(f"serving graph: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(3)
+    out = {"graph": {"scale": args.serve_scale, "edge_factor": 16, "vertices": g.n,
+                     "edges": g.m, "edge_bytes": 8, "resident": True}}
+    # the same 64 uniform random sources: one query per launch sequence
+    # (plain single-source kernels) vs. one 64-lane batch
+    b64 = rng.integers(0, g.n, 64).astype(np.uint32)
+    deg = np.diff(g.offsets()).astype(np.int64)
+    out["sources_isolated"] = int((deg[b64] == 0).sum())
+
+    def plain_qps(fn):
+        fn(int(b64[0]), out=False)
+        t = time.perf_counter()
+        for s in b64:
+            fn(int(s), out=False)
+        return round(64 / (time.perf_counter() - t), 2)
+
+    # plain BFS two ways: the reference's one-query-per-launch top-down
+    # kernel (the serving target's baseline), and this engine's own
+    # direction-optimising single query
+    g.set_option("direction", 0)
+    out["plain_bfs_topdown_qps"] = plain_qps(g.bfs)
+    g.set_option("direction", 1)
+    out["plain_bfs_qps"] = plain_qps(g.bfs)
+    out["plain_sssp_qps"] = plain_qps(g.sssp)
+    batch = {}
+    for algo in ("bfs", "sssp"):
+        r = g.batch(algo, b64)
+        batch[algo] = r
+        out[f"batch64_{algo}_qps"] = round(64 / (r["stats"].ms / 1e3), 2)
+        out[f"batch64_{algo}_levels"] = r["stats"].iterations
+        out[f"batch64_{algo}_vs_plain"] = round(out[f"batch64_{algo}_qps"] /
+                                                out[f"plain_{algo}_qps"], 2)
+    out["batch64_bfs_vs_plain_topdown"] = round(out["batch64_bfs_qps"] /
+                                                out["plain_bfs_topdown_qps"], 2)
+    log(f"plain/batch: {out}")
+    # oracle digests of the 64 sources (BFS: one 64-query multi-source run;
+    # SSSP: delta-stepping per source on the host cores)
+    expect = {}
+    if not args.no_check:
+        t = time.perf_counter()
+        off, dst, w, _ = g.export()
+        og = O.CSR(off, dst, w)
+        ref = O.bfs_multi(og, b64, symmetric=True)
+        for i, s in enumerate(b64):
+            expect[("bfs", int(s))] = (int(ref["hash"][i]), int(ref["reached"][i]),
+                                      int(ref["depth"][i]))
+        for s in b64[:args.check_sssp]:
+            if ("sssp", int(s)) not in expect:
+                expect[("sssp", int(s))] = O.hash_values(O.sssp_delta_mt(og, int(s)), g.n + 1)
+        del og, off, dst, w
+        for algo in ("bfs", "sssp"):
+            r = batch[algo]
+            rows = [i for i, s in enumerate(b64) if (algo, int(s)) in expect]
+            ok = sum((int(r["hash"][i]), int(r["reached"][i]), int(r["depth"][i])) ==
+                     tuple(int(x) for x in expect[(algo, int(b64[i]))]) for i in rows)
+            out[f"batch64_{algo}_oracle"] = {"checked": len(rows), "matches": ok,
+                                             "all_match": ok == len(rows)}
+        out["oracle_check_s"] = round(time.perf_counter() - t, 1)
+        expect = {k: tuple(int(x) for x in v) for k, v in expect.items()}
+        log(f"batch oracle checks: {out['batch64_bfs_oracle']} {out['batch64_sssp_oracle']}")
+
+    def stream(n, mix):
+        """n queries with uniform random sources (50/50 BFS/SSSP when mix);
+        with the checks on, `checked` positions at random carry the batch
+        sources (each checked source twice per algorithm)."""
+        qs = [("bfs" if (not mix or rng.random() < 0.5) else "sssp", int(s))
+              for s in rng.integers(0, g.n, n)]
+        chk = []
+        if expect:
+            algos = ("bfs", "sssp") if mix else ("bfs",)
+            picks = [(a, s) for a, s in expect if a in algos] * 2
+            pos = rng.choice(n, min(n, len(picks)), replace=False)
+            for p, q in zip(pos, picks):
+                qs[int(p)] = q
+            chk = sorted(int(p) for p in pos)
+        return qs, chk
+
+    nq = args.serve_queries
+    qb, cb = stream(nq, mix=False)
+    sat, rs = serve_stream(T, g, qb, None, 4)
+    pois, rp = serve_stream(T, g, qb, 0.7 * sat["qps"], 5)
+    if expect:
+        sat["oracle"] = check_served(rs, qb, cb, expect)
+        pois["oracle"] = check_served(rp, qb, cb, expect)
+    out["serve_bfs_saturated"] = sat
+    out["serve_bfs_poisson70"] = pois
+    out["serve_bfs_speedup_vs_plain"] = round(sat["qps"] / out["plain_bfs_qps"], 2)
+    out["serve_bfs_speedup_vs_plain_topdown"] = round(sat["qps"] /
+                                                      out["plain_bfs_topdown_qps"], 2)
+    log(f"bfs stream: {sat} / {pois}")
+    # the mixed 50/50 BFS+SSSP stream of the configuration
+    qm, cm = stream(nq, mix=True)
+    msat, rs = serve_stream(T, g, qm, None, 6)
+    mpois, rp = serve_stream(T, g, qm, 0.7 * msat["qps"], 7)
+    if expect:
+        msat["oracle"] = check_served(rs, qm, cm, expect)
+        mpois["oracle"] = check_served(rp, qm, cm, expect)
+    out["serve_mix_saturated"] = msat
+    out["serve_mix_poisson70"] = mpois
+    log(f"mix stream: {msat} / {mpois}")
+    g.close()
+    return out
+
+
+def parity_summary(out):
+    """Every full-size check of the run, and whether all passed."""
+    checks = {}
+    if "cpu_matches_gpu" in out:
+        checks["c2_bfs_levels_vs_orc_bfs_mt"] = out["cpu_matches_gpu"]
+    if "oracle_matches" in out.get("cc", {}):
+        checks["c2_cc_labels_vs_orc_cc_mt"] = out["cc"]["oracle_matches"]
+    if "all_match" in out.get("multi_root", {}):
+        checks["c2_64_roots_vs_orc_bfs_multi"] = out["multi_root"]["all_match"]
+    if "cpu_matches_gpu" in out.get("c2_topdown", {}):
+        checks["c2_topdown_levels_vs_orc_bfs_mt"] = out["c2_topdown"]["cpu_matches_gpu"]
+    if "cpu_matches_gpu" in out.get("sssp", {}):
+        checks["c3_sssp_vs_orc_sssp_mt"] = out["sssp"]["cpu_matches_gpu"]
+    for k in ("push", "pull"):
+        if "within_1e-5" in out.get("pr", {}).get(k, {}):
+            checks[f"c4_pr_{k}_l1_within_1e-5"] = out["pr"][k]["within_1e-5"]
+    sv = out.get("serving", {})
+    for k in ("batch64_bfs_oracle", "batch64_sssp_oracle"):
+        if k in sv:
+            checks[f"c5_{k}"] = sv[k]["all_match"]
+    for k in ("serve_bfs_saturated", "serve_bfs_poisson70", "serve_mix_saturated",
+              "serve_mix_poisson70"):
+        if "oracle" in sv.get(k, {}):
+            checks[f"c5_{k}_served_vs_oracle"] = sv[k]["oracle"]["all_match"]
+    return {"checks": checks, "all_pass": all(checks.values()) if checks else None}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edge-factor", type=int, default=32)
     ap.add_argument("--budget", type=float, default=0.25)
     ap.add_argument("--partition-edges", type=int, default=1 << 16)
     ap.add_argument("--source", type=int, default=1)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--topdown-steps", type=int, default=5)
+    ap.add_argument("--sssp-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check-sssp", type=int, default=64,
+                    help="batch SSSP lanes checked against the oracle (1 CPU run each)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-check", action="store_true", help="skip the oracle checks")
     ap.add_argument("--no-cc", action="store_true")
+    ap.add_argument("--no-topdown", action="store_true")
+    ap.add_argument("--no-sssp", action="store_true")
+    ap.add_argument("--no-pr", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-scale", type=int, default=25)
-    ap.add_argument("--serve-queries", type=int, default=2000)
-    ap.add_argument("--workload", choices=["bfs", "sssp", "pr"], default="bfs",
-                    help="bfs: the headline line (BFS + CC + serving); sssp / pr: the "
-                         "SSSP and PageRank configurations, printed as their own line")
+    ap.add_argument("--serve-queries", type=int, default=20000)
+    ap.add_argument("--workload", choices=["all", "bfs", "sssp", "pr", "serve"], default="all",
+                    help="all: the driver line (every block); the others print one block")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -352,83 +644,31 @@ def main():
         sys.exit(2)
 
     import thievory_amd as T
+    O = None if (args.no_cpu and args.no_check) else oracle()
     dev = T.device_info()
     log(f"device {dev}")
-    if args.serve_only:
-        print(json.dumps({"serving": serving_bench(T, args)}), flush=True)
-        return
+    t_start = time.perf_counter()
     if args.workload == "sssp":
-        print(json.dumps(sssp_bench(T, args)), flush=True)
+        print(json.dumps(sssp_block(T, O, args)), flush=True)
         return
     if args.workload == "pr":
-        print(json.dumps(pr_bench(T, args)), flush=True)
+        print(json.dumps(pr_block(T, O, args)), flush=True)
         return
-    t0 = time.perf_counter()
-    g = T.Graph.rmat(args.scale, args.edge_factor, seed=1, symmetric=True)
-    gen_s = time.perf_counter() - t0
-    log(f"generated R-MAT scale {args.scale} ef {args.edge_factor}: n={g.n} m={g.m} "
-        f"in {gen_s:.1f} s")
-    budget = int(args.budget * g.m * g.edge_bytes)
-    t0 = time.perf_counter()
-    g.store_config(budget, args.partition_edges)
-    store = g.store_info()
-    log(f"store: {store} (budget {budget} B) in {time.perf_counter() - t0:.1f} s")
-
-    for i in range(args.warmup):
-        _, st = g.bfs(args.source, out=False)
-        log(f"warmup {i}: {st.ms:.1f} ms, {st.iterations} levels")
-
-    stats = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        _, st = g.bfs(args.source, out=False)
-        stats.append(st)
-        log(f"step {i}: {st.ms:.2f} ms, {st.edges_traversed / st.ms / 1e6:.2f} GTEPS, "
-            f"probe {st.probe_ms:.2f} ms, expand {st.expand_ms:.2f} ms, "
-            f"bottom-up {st.bottom_up_ms:.2f} ms, "
-            f"levels {st.iterations} ({st.bottom_up_levels} bottom-up)")
-    wall = time.perf_counter() - t0
-
-    trav = sum(s.edges_traversed for s in stats)
-    value = trav / wall / 1e9
-    ms_per_step = wall * 1e3 / args.steps
-    workload = f"bfs-rmat{args.scale}-ef{args.edge_factor}-budget{int(args.budget * 100)}pct"
-    roof = bfs_roofline(stats, workload)
-    out = {
-        "metric": METRIC, "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-        "data": "synthetic (seeded R-MAT, generated on the GPU)",
-        "config": {"workload": workload,
-                   "scale": args.scale, "edge_factor": args.edge_factor, "vertices": g.n,
-                   "edges": g.m, "edge_bytes": g.edge_bytes, "budget_frac": args.budget,
-                   "partition_edges": args.partition_edges, "source": args.source,
-                   "store": store, "graph500_gteps": round(value / 2, 4)},
-        "roofline": roof,
-    }
-    if not args.no_cc:
-        cc, cst = g.cc()
-        # union-find CC reads a small part of the edges: the graph's edge
-        # count over the time is the algorithm-independent rate
-        out["cc"] = {"ms": round(cst.ms, 2), "edges_processed": cst.edges_traversed,
-                     "iterations": cst.iterations,
-                     "gteps": round(cst.edges_traversed / cst.ms / 1e6, 3),
-                     "graph_edges_per_s_g": round(g.m / cst.ms / 1e6, 3),
-                     "method": "union-find" if g.get_option("cc_union") else "label propagation",
-                     "host_bytes": cst.host_bytes,
-                     "mixed_roofline_frac": round(
-                         mixed_roofline_ms(cst.hbm_bytes, cst.host_bytes) / cst.ms, 4)}
-        log(f"cc: {out['cc']}")
-    lv, _ = g.bfs(args.source)  # levels for the CPU cross-check (untimed)
-    if not args.no_cpu:
-        cb, cpu_lv = cpu_baseline(g, args.source, args.cpu_seconds,
-                                  f"R-MAT {args.scale}/{args.edge_factor}")
-        out["cpu_baseline"] = cb
-        out["cpu_matches_gpu"] = bool(np.array_equal(cpu_lv, lv))
-        log(f"cpu baseline: {cb}")
-    g.close()
-    if not args.no_serve:
-        out["serving"] = serving_bench(T, args)
+    if args.workload == "serve":
+        print(json.dumps({"serving": serving_block(T, O, args)}), flush=True)
+        return
+    out = c2_block(T, O, args)
+    if args.workload == "all":
+        if not args.no_sssp:
+            out["sssp"] = sssp_block(T, O, args)
+            log(f"sssp: {out['sssp']}")
+        if not args.no_pr:
+            out["pr"] = pr_block(T, O, args)
+            log(f"pr: {out['pr']}")
+        if not args.no_serve:
+            out["serving"] = serving_block(T, O, args)
+    out["parity"] = parity_summary(out)
+    out["run_s"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -188,7 +188,9 @@ int thv_graph_get_option(const thv_graph *g, int option, double *value);
 int thv_graph_export(const thv_graph *g, uint64_t *off, uint32_t *dst,
                      uint32_t *w, uint32_t *outdeg);
 
-void thv_graph_close(thv_graph *g);
+/* Frees the graph.  THV_EBUSY (and the graph stays open) while a server
+ * runs on it. */
+int thv_graph_close(thv_graph *g);
 
 /* ---- split edge store ------------------------------------------------- */
 
@@ -267,7 +269,12 @@ int thv_serve_start(thv_graph *g, const thv_serve_cfg *cfg, thv_server **out);
 /* Thread-safe; returns THV_EBUSY when the admission queue is full. */
 int thv_serve_submit(thv_server *s, int algo, uint32_t src, uint64_t *ticket);
 /* Thread-safe; status 1 while pending.  full (nullable, n entries) receives
- * the value array when the server keeps them. */
+ * the value array when the server keeps them.  A finished query (status 0
+ * or an error) is returned once: the poll that returns it releases its
+ * entry and value array, and a later poll of the ticket is THV_EINVAL.
+ * With keep_full, at most max(64, 4 GiB / (4 n)) value arrays (finished and
+ * unpolled, plus queries still running) exist at once; submit returns
+ * THV_EBUSY beyond that. */
 int thv_serve_poll(thv_server *s, uint64_t ticket, thv_result *res,
                    uint32_t *full);
 /* Blocks until every submitted query has finished. */

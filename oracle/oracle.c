@@ -711,6 +711,171 @@ ORC_EXPORT uint64_t orc_sssp_mt(uint64_t n, const uint64_t *off,
   return relax_mt(n, off, adj, w, val, cur);
 }
 
+/* SSSP by delta-stepping (Meyer & Sanders 2003) on the host cores: the
+ * pending vertices whose value is below the bucket bound thr have their
+ * edges relaxed with an atomic min (every improvement makes the target
+ * pending again); when none is below thr, thr becomes the least pending
+ * value + delta.  Same fixpoint as orc_sssp / orc_sssp_mt (shortest
+ * distances are unique), with fewer edges relaxed twice.  Returns the number
+ * of edges relaxed. */
+ORC_EXPORT uint64_t orc_sssp_delta_mt(uint64_t n, const uint64_t *off,
+                                      const uint32_t *adj, const uint32_t *w,
+                                      uint32_t src, uint32_t delta, uint32_t *val) {
+  uint8_t *pend = (uint8_t *)calloc(n ? n : 1, 1), *cur = (uint8_t *)calloc(n ? n : 1, 1);
+#pragma omp parallel for schedule(static)
+  for (long long v = 0; v < (long long)n; v++) val[v] = (uint32_t)(n + 1);
+  uint64_t nproc = 0;
+  if (src < n) {
+    val[src] = 1;
+    pend[src] = 1;
+    uint64_t thr = 1 + (uint64_t)(delta ? delta : 1);
+    for (;;) {
+      int any = 0;
+#pragma omp parallel for schedule(static) reduction(| : any)
+      for (long long v = 0; v < (long long)n; v++) {
+        if (pend[v] && val[v] < thr) {
+          pend[v] = 0;
+          cur[v] = 1;
+          any = 1;
+        }
+      }
+      if (!any) {
+        uint32_t lo = 0xFFFFFFFFu;
+#pragma omp parallel for schedule(static) reduction(min : lo)
+        for (long long v = 0; v < (long long)n; v++)
+          if (pend[v] && val[v] < lo) lo = val[v];
+        if (lo == 0xFFFFFFFFu) break;
+        thr = (uint64_t)lo + (delta ? delta : 1);
+        continue;
+      }
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : nproc)
+      for (long long v = 0; v < (long long)n; v++) {
+        if (!cur[v]) continue;
+        cur[v] = 0;
+        uint32_t base = __atomic_load_n(&val[v], __ATOMIC_RELAXED);
+        nproc += off[v + 1] - off[v];
+        for (uint64_t i = off[v]; i < off[v + 1]; i++) {
+          uint64_t nv = (uint64_t)base + w[i];
+          if (nv >= 0xFFFFFFFFull) continue;
+          if (atomic_min_u32(&val[adj[i]], (uint32_t)nv))
+            __atomic_store_n(&pend[adj[i]], 1, __ATOMIC_RELAXED);
+        }
+      }
+    }
+  }
+  free(pend);
+  free(cur);
+  return nproc;
+}
+
+/* k <= 64 BFS queries at once on the host cores (multi-source BFS with one
+ * bit per query, Then et al., VLDB 2014): seen[v] holds the queries that
+ * reached v, F[v] those with v in their frontier.  A top-down level hands
+ * each out-neighbour w the queries F[v] & ~seen[w]; when symmetric != 0 and
+ * the frontier's edges exceed 1/8 of the edges still unsettled, a level
+ * runs bottom-up instead: every vertex ORs the frontier masks of its
+ * neighbours (its in-neighbours, on a symmetric graph) and stops when it has
+ * every query it misses.  Either way query q reaches w at level L+1 exactly
+ * when w is unreached by q and adjacent to q's level-L frontier, so each
+ * query's values are those of orc_bfs.  Per query: the digest of
+ * orc_hash_values (sentinel n+1), the reached count and the largest value;
+ * out_full (nullable, k x n, query-major) gets the value arrays. */
+ORC_EXPORT void orc_bfs_multi(uint64_t n, const uint64_t *off, const uint32_t *adj,
+                              const uint32_t *srcs, uint32_t k, int symmetric,
+                              uint64_t *out_hash, uint64_t *out_reached,
+                              uint32_t *out_depth, uint32_t *out_full) {
+  if (k > 64) k = 64;
+  uint64_t *seen = (uint64_t *)calloc(n ? n : 1, 8);
+  uint64_t *F = (uint64_t *)calloc(n ? n : 1, 8);
+  uint64_t *N = (uint64_t *)calloc(n ? n : 1, 8);
+  uint64_t hash[64] = {0}, reached[64] = {0};
+  uint32_t depth[64] = {0};
+  const uint64_t all = k == 64 ? ~0ull : ((1ull << k) - 1);
+  if (out_full) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)(k * n); i++) out_full[i] = (uint32_t)(n + 1);
+  }
+  for (uint32_t q = 0; q < k; q++) {
+    if (srcs[q] >= n) continue;
+    seen[srcs[q]] |= 1ull << q;
+    F[srcs[q]] |= 1ull << q;
+    hash[q] = orc_mix64(((uint64_t)srcs[q] << 32) | 1u);
+    reached[q] = 1;
+    depth[q] = 1;
+    if (out_full) out_full[(uint64_t)q * n + srcs[q]] = 1;
+  }
+  uint64_t m = off[n], settled = 0;  /* edges of vertices every query reached */
+  for (uint32_t level = 1;; level++) {
+    uint64_t fe = 0;
+#pragma omp parallel for schedule(static) reduction(+ : fe)
+    for (long long v = 0; v < (long long)n; v++)
+      if (F[v]) fe += off[v + 1] - off[v];
+    const int bu = symmetric && fe > (m - settled) / 8;
+    if (bu) {
+#pragma omp parallel for schedule(dynamic, 1024)
+      for (long long wv = 0; wv < (long long)n; wv++) {
+        const uint64_t miss = all & ~seen[wv];
+        uint64_t acc = 0;
+        if (miss)
+          for (uint64_t i = off[wv]; i < off[wv + 1] && (acc & miss) != miss; i++)
+            acc |= F[adj[i]];
+        N[wv] = acc & miss;
+      }
+    } else {
+#pragma omp parallel for schedule(dynamic, 1024)
+      for (long long v = 0; v < (long long)n; v++) {
+        const uint64_t f = F[v];
+        if (!f) continue;
+        for (uint64_t i = off[v]; i < off[v + 1]; i++) {
+          const uint32_t wv = adj[i];
+          const uint64_t add = f & ~seen[wv];
+          if (add && (__atomic_load_n(&N[wv], __ATOMIC_RELAXED) & add) != add)
+            __atomic_fetch_or(&N[wv], add, __ATOMIC_RELAXED);
+        }
+      }
+    }
+    int any = 0;
+    uint64_t newly_settled = 0;
+#pragma omp parallel reduction(| : any) reduction(+ : newly_settled)
+    {
+      uint64_t h[64] = {0}, r[64] = {0};
+#pragma omp for schedule(static)
+      for (long long wv = 0; wv < (long long)n; wv++) {
+        const uint64_t gain = N[wv] & ~seen[wv];
+        N[wv] = 0;
+        F[wv] = gain;
+        if (!gain) continue;
+        any = 1;
+        seen[wv] |= gain;
+        if ((seen[wv] & all) == all) newly_settled += off[wv + 1] - off[wv];
+        const uint64_t hv = orc_mix64(((uint64_t)wv << 32) | (level + 1));
+        for (uint64_t g = gain; g; g &= g - 1) {
+          const int q = __builtin_ctzll(g);
+          h[q] += hv;
+          r[q]++;
+          if (out_full) out_full[(uint64_t)q * n + wv] = level + 1;
+        }
+      }
+#pragma omp critical
+      for (uint32_t q = 0; q < k; q++) {
+        hash[q] += h[q];
+        reached[q] += r[q];
+        if (r[q] && depth[q] < level + 1) depth[q] = level + 1;
+      }
+    }
+    settled += newly_settled;
+    if (!any) break;
+  }
+  for (uint32_t q = 0; q < k; q++) {
+    if (out_hash) out_hash[q] = hash[q];
+    if (out_reached) out_reached[q] = reached[q];
+    if (out_depth) out_depth[q] = depth[q];
+  }
+  free(seen);
+  free(F);
+  free(N);
+}
+
 /* PageRank pull (Jacobi over the active set) on the CSC, in FP32 ranks like
  * the GPU path; returns the iteration count. */
 ORC_EXPORT int orc_pr_pull_mt(uint64_t n, const uint64_t *ioff,

@@ -63,6 +63,11 @@ def lib():
         L.orc_sssp_bf.argtypes = [C.c_uint64, u64p, u32p, u32p, C.c_uint32, u32p]
         L.orc_sssp_mt.restype = C.c_uint64
         L.orc_sssp_mt.argtypes = [C.c_uint64, u64p, u32p, u32p, C.c_uint32, u32p]
+        L.orc_sssp_delta_mt.restype = C.c_uint64
+        L.orc_sssp_delta_mt.argtypes = [C.c_uint64, u64p, u32p, u32p, C.c_uint32,
+                                        C.c_uint32, u32p]
+        L.orc_bfs_multi.argtypes = [C.c_uint64, u64p, u32p, u32p, C.c_uint32, C.c_int,
+                                    u64p, u64p, u32p, C.c_void_p]
         L.orc_pr_exact.restype = C.c_int
         L.orc_pr_exact.argtypes = [C.c_uint64, u64p, u32p, C.c_double, C.c_double,
                                    C.c_int, f64p]
@@ -249,6 +254,29 @@ def sssp_mt(g, src):
     out = np.zeros(g.n, np.uint32)
     lib().orc_sssp_mt(g.n, g.off, g.adj, _weights(g), src, out)
     return out
+
+
+def sssp_delta_mt(g, src, delta=16):
+    """Delta-stepping SSSP on the host cores (same values as sssp)."""
+    out = np.zeros(g.n, np.uint32)
+    lib().orc_sssp_delta_mt(g.n, g.off, g.adj, _weights(g), src, delta, out)
+    return out
+
+
+def bfs_multi(g, srcs, symmetric=False, full=False):
+    """k <= 64 BFS queries at once (one bit per query): per-query digests
+    (hash_values with sentinel n+1), reached counts, depths, and optionally
+    the k x n value arrays."""
+    srcs = np.ascontiguousarray(srcs, np.uint32)
+    k = len(srcs)
+    if k > 64:
+        raise ValueError("at most 64 queries")
+    h = np.zeros(max(k, 1), np.uint64)
+    r = np.zeros(max(k, 1), np.uint64)
+    d = np.zeros(max(k, 1), np.uint32)
+    f = np.zeros((k, g.n), np.uint32) if full else None
+    lib().orc_bfs_multi(g.n, g.off, g.adj, srcs, k, int(bool(symmetric)), h, r, d, _ptr(f))
+    return {"hash": h[:k], "reached": r[:k], "depth": d[:k], "full": f}
 
 
 ALPHA_F32 = float(np.float32(0.85))

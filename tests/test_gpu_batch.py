@@ -252,6 +252,38 @@ def test_serving_errors():
     g.store_config(None, 128)
 
 
+def test_serving_releases_results_and_pins_graph():
+    """A finished query is handed over once (a second poll of its ticket is
+    an error, so a long stream does not keep every result); while a server
+    runs, closing the graph and autotuning are refused (THV_EBUSY) and the
+    graph stays usable; after stop both work."""
+    og = O.rmat(10, 8, seed=4, weights=True)
+    g = T.Graph.from_arrays(og.off, og.adj, og.w)
+    srv = T.Server(g, max_lanes=4, keep_full=True)
+    t = srv.submit("bfs", 3)
+    srv.drain()
+    full = np.zeros(og.n, np.uint32)
+    assert srv.poll(t, full)["status"] == 0
+    assert np.array_equal(full, O.bfs(og, 3))
+    with pytest.raises(T.ThvError) as e:
+        srv.poll(t)
+    assert e.value.code == -1
+    with pytest.raises(T.ThvError) as e:
+        g.close()
+    assert e.value.code == -6
+    with pytest.raises(T.ThvError) as e:
+        g.autotune()
+    assert e.value.code == -6
+    t = srv.submit("sssp", 5)       # still serving after the refusals
+    srv.drain()
+    r = srv.poll(t)
+    assert r["status"] == 0
+    assert (r["hash"], r["reached"], r["depth"]) == O.hash_values(O.sssp(og, 5), og.n + 1)
+    srv.stop()
+    g.autotune()
+    g.close()
+
+
 def test_autotune_keeps_results_exact(tmp_path, g14):
     g, og = g14
     g.store_config(int(0.25 * g.m * 8), 1 << 12)

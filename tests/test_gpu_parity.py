@@ -298,6 +298,28 @@ def test_parent_hints(rmat16, rmat16u):
     assert gd.get_option("hint_bytes") == 0
 
 
+def test_side_area_near_minimum_hint_budget():
+    """A bottom-up layout budget just above the hints' minimum (8n bytes):
+    the short-list side area cannot fit even with every partition evicted,
+    so the store keeps its partitions and goes without the side area (it
+    used to evict them all); BFS stays exact."""
+    _say("case: side area at the minimum hint budget")
+    og = O.rmat(16, 16, seed=1)
+    g = T.Graph.from_arrays(og.off, og.adj)
+    g.set_option("symmetric", 1)
+    budget = 8 * og.n + (64 << 10)
+    deg = np.diff(og.off).astype(np.int64)
+    short = int(deg[(deg >= 1) & (deg <= 16)].sum())
+    assert short * 4 > budget - 4 * og.n     # the side area alone exceeds the room
+    g.store_config(budget, 1024)
+    assert g.get_option("hint_bytes") == 4 * og.n
+    info = g.store_info()
+    assert info["device_edges"] > 0
+    assert info["device_edges"] * 4 <= budget - 4 * og.n
+    for s in (1, 7, 999):
+        assert np.array_equal(g.bfs(s)[0], O.bfs(og, s)), s
+
+
 def test_symmetric_hint_on_arrays():
     og = O.rmat(12, 16, seed=11)
     g = T.Graph.from_arrays(og.off, og.adj)

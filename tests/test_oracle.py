@@ -195,6 +195,61 @@ def test_bfs_sssp_vs_scipy(rmat12w, golden):
             assert np.array_equal(O.bfs_mt(g, s)[0], O.bfs(g, s))
 
 
+@pytest.mark.parametrize("delta", [1, 7, 16, 1000])
+def test_sssp_delta_mt_vs_scipy(rmat12w, golden, delta):
+    """Delta-stepping on the host cores: every bucket width gives scipy's
+    Dijkstra distances (clipped at n+1), on the R-MAT graph and the
+    converter's random fixture (kept duplicates and self-loops)."""
+    for g in (rmat12w, O.read_graph(os.path.join(golden, "rand64.csr"), "csr")):
+        Aw = _scipy_matrix(g, weights=True)
+        for s in (0, 1, 3, g.n - 1):
+            exp = _clip(csg.dijkstra(Aw, indices=s), g.n)
+            assert np.array_equal(O.sssp_delta_mt(g, s, delta), exp), (g.n, s, delta)
+
+
+def test_sssp_delta_mt_clips_at_sentinel():
+    """A path 0 -> 1 -> 2 -> 3 with weight 2: 1 + dist passes n + 1 = 5 at
+    vertex 2, which then stays at the sentinel (and so does vertex 3 behind
+    it), as the strict-less relaxation from val[src] = 1 leaves it."""
+    g = O.CSR(np.array([0, 1, 2, 3, 3], np.uint64), np.array([1, 2, 3], np.uint32),
+              np.array([2, 2, 2], np.uint32))
+    assert O.sssp_delta_mt(g, 0, 16).tolist() == [1, 3, 5, 5]
+    assert O.sssp(g, 0).tolist() == [1, 3, 5, 5]
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_bfs_multi_vs_scipy(symmetric):
+    """64 queries at once: each query's value array equals scipy's unweighted
+    shortest paths (+1, clipped at n+1), and its digest, reached count and
+    depth equal hash_values of that array.  Directed R-MAT runs top-down
+    levels only; the symmetric one also bottom-up levels."""
+    g = O.rmat(12, 16, seed=9, symmetric=symmetric)
+    A = _scipy_matrix(g, weights=False)
+    rng = np.random.default_rng(2)
+    srcs = rng.integers(0, g.n, 64).astype(np.uint32)
+    srcs[1] = srcs[2]                                     # a repeated source
+    srcs[3] = int(np.flatnonzero(np.diff(g.off) == 0)[0])  # an isolated one
+    out = O.bfs_multi(g, srcs, symmetric=symmetric, full=True)
+    for q, s in enumerate(srcs):
+        exp = _clip(csg.shortest_path(A, indices=int(s), unweighted=True), g.n)
+        assert np.array_equal(out["full"][q], exp), (q, s)
+        h, r, d = O.hash_values(exp.astype(np.uint32), g.n + 1)
+        assert (out["hash"][q], out["reached"][q], out["depth"][q]) == (h, r, d), (q, s)
+
+
+def test_bfs_multi_small_k_and_ragged_n():
+    """k below 64 and vertex counts that are not multiples of 64: equal to
+    the serial queue BFS query by query."""
+    rng = np.random.default_rng(5)
+    for n in (1, 5, 65, 200):
+        g, _ = _random_small(rng, n, 0.05) if n > 1 else _random_small(rng, 1, 0.0)
+        for k in (1, 3, 17):
+            srcs = rng.integers(0, n, k).astype(np.uint32)
+            out = O.bfs_multi(g, srcs, full=True)
+            for q, s in enumerate(srcs):
+                assert np.array_equal(out["full"][q], O.bfs(g, int(s))), (n, k, q)
+
+
 def test_cc_vs_scipy_symmetric(rmat12w):
     g = rmat12w
     ncomp, lab = csg.connected_components(_scipy_matrix(g, False), directed=False)

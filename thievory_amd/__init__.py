@@ -98,7 +98,7 @@ def lib():
         "thv_graph_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "thv_graph_export": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
-        "thv_graph_close": (None, [_vp]),
+        "thv_graph_close": (C.c_int, [_vp]),
         "thv_graph_set_option": (C.c_int, [_vp, C.c_int, C.c_double]),
         "thv_graph_get_option": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
         "thv_store_config": (C.c_int, [_vp, C.c_uint64, C.c_uint32]),
@@ -203,7 +203,7 @@ class Graph:
 
     def close(self):
         if self._h:
-            lib().thv_graph_close(self._h)
+            check(lib().thv_graph_close(self._h))
             self._h = None
 
     def __del__(self):

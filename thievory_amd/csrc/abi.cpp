@@ -179,12 +179,21 @@ int thv_graph_export(const thv_graph *gh, uint64_t *off, uint32_t *dst, uint32_t
   });
 }
 
-void thv_graph_close(thv_graph *gh) {
-  if (!gh) return;
+int thv_graph_close(thv_graph *gh) {
+  if (!gh) return THV_OK;
+  Graph *g = reinterpret_cast<Graph *>(gh);
+  {
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (g->servers.load() > 0) {
+      set_last_error("a server is running on this graph: stop it first");
+      return THV_EBUSY;
+    }
+  }
   try {
-    delete reinterpret_cast<Graph *>(gh);
+    delete g;
   } catch (...) {
   }
+  return THV_OK;
 }
 
 int thv_store_config(thv_graph *gh, uint64_t budget, uint32_t partition_edges) {
@@ -268,6 +277,7 @@ int thv_autotune(thv_graph *gh, const char *cache_path, int *chosen) {
     NEED(gh);
     Graph *g = reinterpret_cast<Graph *>(gh);
     std::lock_guard<std::mutex> lk(g->mu);
+    if (g->servers.load()) fail(THV_EBUSY, "a server is running on this graph");
     autotune(*g, cache_path, chosen);
   });
 }

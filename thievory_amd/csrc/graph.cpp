@@ -257,16 +257,30 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   if (sparse_first && budget != UINT64_MAX) {
     for (uint32_t p = 0; p < s.P; p++)
       if (!s.pres[p]) side_edges += short_edges(p);
-    while (used + side_edges * rec > budget && !chosen.empty()) {
-      const uint32_t p = chosen.back();
-      chosen.pop_back();
-      s.pres[p] = 0;
-      used -= pedges(p) * rec;
-      s.dev_edges -= pedges(p);
-      s.Pdev--;
-      side_edges += short_edges(p);
+    // how many of the last partitions taken must make room; if the side
+    // area does not fit even with none of them, keep them all and go
+    // without the side area
+    uint64_t u = used, se = side_edges;
+    size_t evict = 0;
+    while (u + se * rec > budget && evict < chosen.size()) {
+      const uint32_t p = chosen[chosen.size() - 1 - evict];
+      u -= pedges(p) * rec;
+      se += short_edges(p);
+      evict++;
     }
-    if (used + side_edges * rec > budget) side_edges = 0;  // no room at all
+    if (u + se * rec > budget) {
+      side_edges = 0;  // no room at all: the partitions stay
+    } else {
+      for (size_t i = 0; i < evict; i++) {
+        const uint32_t p = chosen.back();
+        chosen.pop_back();
+        s.pres[p] = 0;
+        s.dev_edges -= pedges(p);
+        s.Pdev--;
+      }
+      used = u;
+      side_edges = se;
+    }
   }
 
   // Slab: resident partitions packed in vertex order; contiguous runs of

@@ -50,6 +50,11 @@ struct Server {
   std::string errmsg;
   uint64_t t0 = 0;
   static constexpr size_t MAX_WAITING = 1u << 20;
+  // keep_full: finished value arrays wait on the host until their poll; at
+  // most max(KEEP_MIN, KEEP_BYTES / (4 n)) of them (counting the queries
+  // still running) exist at once, beyond that submit returns THV_EBUSY
+  static constexpr uint64_t KEEP_MIN = 64, KEEP_BYTES = 4ull << 30;
+  uint64_t keep_cap = 0;
   // BFS cohorts: two engines alternate.  While SSSP lanes are running (a
   // mixed load, where SSSP levels slow every cohort down) the other engines
   // also start cohorts, for queries that have waited SPILL_MS already.  When
@@ -190,6 +195,12 @@ int thv_serve_start(thv_graph *gh, const thv_serve_cfg *cfg, thv_server **out) {
     std::unique_ptr<Server> s(new Server());
     s->g = g;
     s->cfg = *cfg;
+    // the store must not change while the engines are built on it: hold the
+    // graph's lock until the server is counted (store_config, set_option,
+    // autotune and close refuse while it runs)
+    std::lock_guard<std::mutex> glk(g->mu);
+    s->keep_cap = std::max<uint64_t>(Server::KEEP_MIN,
+                                     Server::KEEP_BYTES / std::max<uint64_t>(1, g->n * 4));
     uint32_t lanes = cfg->max_lanes ? std::min<uint32_t>(cfg->max_lanes, 64) : 64;
     uint64_t budget = cfg->scratch_bytes;
     if (budget == 0) {
@@ -238,6 +249,8 @@ int thv_serve_submit(thv_server *sh, int algo, uint32_t src, uint64_t *ticket) {
     if (s->err) fail(THV_ESTATE, "server failed: " + s->errmsg);
     if (s->stop) fail(THV_ESTATE, "server is stopping");
     if (s->waiting[a].size() >= Server::MAX_WAITING) fail(THV_EBUSY, "admission queue full");
+    if (s->cfg.keep_full && s->outstanding + s->full.size() >= s->keep_cap)
+      fail(THV_EBUSY, "kept value arrays at their bound: poll finished queries first");
     const uint64_t t = s->next_ticket++;
     thv_result r;
     memset(&r, 0, sizeof r);
@@ -261,12 +274,17 @@ int thv_serve_poll(thv_server *sh, uint64_t ticket, thv_result *out, uint32_t *f
     if (!s || !out) fail(THV_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->res.find(ticket);
-    if (it == s->res.end()) fail(THV_EINVAL, "unknown ticket");
+    if (it == s->res.end()) fail(THV_EINVAL, "unknown ticket (never issued, or already returned)");
     *out = it->second;
     if (fullv && it->second.status == 0) {
       auto f = s->full.find(ticket);
       if (f == s->full.end()) fail(THV_ESTATE, "server does not keep value arrays");
       memcpy(fullv, f->second.data(), f->second.size() * 4);
+    }
+    // a finished query is handed over once: its entry and value array go
+    if (it->second.status != 1) {
+      s->full.erase(ticket);
+      s->res.erase(it);
     }
     return THV_OK;
   });
