@@ -213,6 +213,21 @@ int thv_store_info(const thv_graph *g, uint32_t *partitions,
 /* BFS levels (BFS32, bfs/bfs.cu:8-542).  out: n x u32 host buffer. */
 int thv_bfs(thv_graph *g, uint32_t src, uint32_t *out, thv_stats *st);
 
+/* BFS with no allocation (no counterpart in the reference; for callers that
+ * cannot allocate: code under stream capture, tight serving loops).  The
+ * same levels as thv_bfs, by top-down levels only, with every per-call
+ * buffer (queues, tile tables, bitmaps, values, counters) carved from one
+ * static device array of THV_NOALLOC_SCRATCH_BYTES per GPU, which the GPU
+ * holds from library load to process end.  The call allocates no device or
+ * pinned memory.  The graph's store must be built already (thv_store_config
+ * or an earlier algorithm call), else THV_ESTATE.  The workspace takes about
+ * 30 bytes per vertex plus 4 per partition and 8 per 2048 edges; a graph
+ * that needs more than the array is THV_EINVAL.  Every call on one GPU
+ * shares its array: calls that overlap on one GPU race.  out: n x u32 host
+ * buffer; st as for thv_bfs (expand_ms stays 0: no events are created). */
+#define THV_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int thv_bfs_noalloc(thv_graph *g, uint32_t src, uint32_t *out, thv_stats *st);
+
 /* Min-label propagation (CC32, cc/cc.cu:71-614). */
 int thv_cc(thv_graph *g, uint32_t *out, thv_stats *st);
 

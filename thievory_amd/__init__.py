@@ -105,6 +105,7 @@ def lib():
         "thv_store_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint64)]),
         "thv_bfs": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(Stats)]),
+        "thv_bfs_noalloc": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(Stats)]),
         "thv_cc": (C.c_int, [_vp, _vp, C.POINTER(Stats)]),
         "thv_sssp": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(Stats)]),
         "thv_pr": (C.c_int, [_vp, C.c_int, C.c_float, C.c_float, C.c_int, _vp,
@@ -255,6 +256,14 @@ class Graph:
         out = _out(out, self.n, np.uint32)
         st = Stats()
         check(lib().thv_bfs(self._h, src, _p(out), C.byref(st)))
+        return out, st
+
+    def bfs_noalloc(self, src, out=None):
+        """BFS levels with every per-call buffer in the library's static
+        device array (thv_bfs_noalloc): the call allocates nothing."""
+        out = _out(out, self.n, np.uint32)
+        st = Stats()
+        check(lib().thv_bfs_noalloc(self._h, src, _p(out), C.byref(st)))
         return out, st
 
     def cc(self, out=None):

@@ -230,6 +230,15 @@ int thv_bfs(thv_graph *gh, uint32_t src, uint32_t *out, thv_stats *st) {
   });
 }
 
+int thv_bfs_noalloc(thv_graph *gh, uint32_t src, uint32_t *out, thv_stats *st) {
+  return guarded([&] {
+    NEED(gh);
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_bfs_noalloc(*g, src, out, st);
+  });
+}
+
 int thv_cc(thv_graph *gh, uint32_t *out, thv_stats *st) {
   return guarded([&] {
     NEED(gh);

@@ -181,20 +181,22 @@ static void stats_level(Graph &g, thv_stats *st, const LevelCtr &h) {
   st->iterations++;
 }
 
-static ExpandArgs base_args(Graph &g) {
-  ExpandArgs a = expand_args(g, g.ws.qs);
-  a.val = g.ws.val.as<uint32_t>();
-  a.vis = g.ws.vis.as<uint32_t>();
+static ExpandArgs base_args(Graph &g, Workspace &w) {
+  ExpandArgs a = expand_args(g, w.qs);
+  a.val = w.val.as<uint32_t>();
+  a.vis = w.vis.as<uint32_t>();
   return a;
 }
+static ExpandArgs base_args(Graph &g) { return base_args(g, g.ws); }
 
 // Level-synchronous loop shared by BFS, CC and SSSP.  front[0] holds the
-// first frontier on entry.
-static void value_loop(Graph &g, Algo algo, thv_stats *st) {
-  Workspace &w = g.ws;
+// first frontier on entry.  Without events (w.ev null) no kernel time is
+// taken.
+static void value_loop(Graph &g, Workspace &w, Algo algo, thv_stats *st) {
   hipStream_t s = g.stream;
   const uint64_t words = g.n / 32 + 2;
-  ExpandArgs a = base_args(g);
+  const bool timed = w.ev[0] && w.ev[1];
+  ExpandArgs a = base_args(g, w);
   LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
   int cur = 0;
   double expand_ms = 0;
@@ -209,16 +211,16 @@ static void value_loop(Graph &g, Algo algo, thv_stats *st) {
     a.ctr = c;
     a.next = w.front[cur ^ 1].as<uint32_t>();
     a.level_value = L + 2;
-    HIPC(hipEventRecord(w.ev[0], s));
+    if (timed) HIPC(hipEventRecord(w.ev[0], s));
     launch_expand(g, algo, a, s);
-    HIPC(hipEventRecord(w.ev[1], s));
+    if (timed) HIPC(hipEventRecord(w.ev[1], s));
     HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const LevelCtr &h = w.h_ctr[L & 1];
     // every launch is counted, the last one (empty frontier, exits at once)
     // included, so launches and time match a kernel trace of the same run
     float ms = 0;
-    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    if (timed) HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
     expand_ms += ms;
     launches++;
     if (h.qn[0] + h.qn[1] == 0) break;
@@ -597,8 +599,26 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
   HIPC(hipStreamSynchronize(s));
   if (g.sym && g.dir_opt) bfs_direction_optimizing(g, st);
-  else value_loop(g, A_BFS, st);
+  else value_loop(g, g.ws, A_BFS, st);
   finish_values(g, out, st, (uint32_t)(g.n + 1));
+}
+
+// The reference's top-down BFS (bfs/bfs.cu:55-526: every level expands the
+// whole frontier) over the caller's workspace: fills the values and the
+// bitmaps with kernels and leaves the levels in w.val.
+void bfs_topdown_in(Graph &g, Workspace &w, uint32_t src, thv_stats *st) {
+  check_src(g, src);
+  hipStream_t s = g.stream;
+  const uint64_t words = bitmap_words(g.n);
+  stats_begin(st);
+  launch_fill_u32(w.val.as<uint32_t>(), (uint32_t)(g.n + 1), g.n, s);
+  launch_fill_u32(w.val.as<uint32_t>() + src, 1u, 1, s);
+  launch_fill_u32(w.vis.as<uint32_t>(), 0u, words, s);
+  launch_fill_u32(w.front[0].as<uint32_t>(), 0u, words, s);
+  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.vis.as<uint32_t>(), src);
+  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
+  HIPC(hipStreamSynchronize(s));
+  value_loop(g, w, A_BFS, st);
 }
 
 void run_cc(Graph &g, uint32_t *out, thv_stats *st) {
@@ -625,7 +645,7 @@ void run_cc(Graph &g, uint32_t *out, thv_stats *st) {
   hipLaunchKernelGGL(k_bits_prefix, dim3(fill_grid(words)), dim3(256), 0, s,
                      w.front[0].as<uint32_t>(), g.n, words);
   HIPC(hipStreamSynchronize(s));
-  value_loop(g, A_CC, st);
+  value_loop(g, g.ws, A_CC, st);
   finish_values(g, out, st, 0xFFFFFFFFu);
 }
 
@@ -704,7 +724,7 @@ void run_sssp(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
   HIPC(hipStreamSynchronize(s));
   if (g.sssp_delta) sssp_delta(g, st);
-  else value_loop(g, A_SSSP, st);
+  else value_loop(g, g.ws, A_SSSP, st);
   finish_values(g, out, st, (uint32_t)(g.n + 1));
 }
 

@@ -52,9 +52,10 @@ void DevMem::alloc(size_t b) {
 }
 
 void DevMem::release() {
-  if (p) (void)hipFree(p);
+  if (p && own) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  own = true;
 }
 
 // Non-coherent (coarse-grained) mapped memory lets the GPU cache the

@@ -46,11 +46,19 @@ void hip_fail(hipError_t e, const char *what, const char *file, int line);
 struct DevMem {
   void *p = nullptr;
   size_t bytes = 0;
+  bool own = true;  // false: a view of memory someone else holds (borrow)
   DevMem() = default;
   DevMem(const DevMem &) = delete;
   DevMem &operator=(const DevMem &) = delete;
   ~DevMem() { release(); }
   void alloc(size_t b);
+  // points at b bytes the caller holds; release() then forgets them
+  void borrow(void *q, size_t b) {
+    release();
+    p = q;
+    bytes = b;
+    own = false;
+  }
   void release();
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
@@ -304,6 +312,11 @@ constexpr uint32_t PR_HUBS = 2048;       // hub table entries
 constexpr uint32_t PR_HUB_SLOTS = 4096;  // LDS hash slots (power of two)
 void ensure_pr_hubs(Graph &g);
 void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
+// Top-down BFS levels over a workspace the caller lays out (no allocation
+// here); w.h_ctr must point at 2 host LevelCtr, w.ev may be null.
+void bfs_topdown_in(Graph &g, Workspace &w, uint32_t src, thv_stats *st);
+// noalloc.hip: BFS with its workspace in the library's static device array
+void run_bfs_noalloc(Graph &g, uint32_t src, uint32_t *out, thv_stats *st);
 void run_cc(Graph &g, uint32_t *out, thv_stats *st);
 // cc.hip: components of a symmetric graph into ws.val; false when a link
 // hit its step guard (the caller then uses label propagation)
