@@ -62,6 +62,21 @@ def mixed_roofline_ms(hbm_bytes, host_bytes):
     return max(hbm_bytes / HBM_PEAK, host_bytes / LINK_PEAK) * 1e3
 
 
+def tuned(r, seconds):
+    """The tuner's choice (thv_tune) as reported: the settings, the median
+    workload time with the starting and the chosen settings, and the
+    candidates timed."""
+    return {"expansion": {"blocks_per_cu": r["blocks_per_cu"],
+                          "edges_per_thread": r["edges_per_thread"]},
+            "probe_blocks_per_cu": r["probe_blocks_per_cu"],
+            "batch_blocks_per_cu": r["batch_blocks_per_cu"],
+            "partition_edges": r["partition_edges"], "sssp_delta": r["sssp_delta"],
+            "pull_degree": r["pull_degree"], "default_ms": round(r["default_ms"], 4),
+            "tuned_ms": round(r["tuned_ms"], 4),
+            "speedup": round(r["default_ms"] / r["tuned_ms"], 4) if r["tuned_ms"] else None,
+            "candidates": r["candidates"], "tuning_s": round(seconds, 1)}
+
+
 def cpu_meta(O):
     return {"cores": O.threads(), "omp_threads": O.threads(),
             "hardware_concurrency": os.cpu_count()}
@@ -250,6 +265,14 @@ def c2_block(T, O, args):
     store = g.store_info()
     store["hint_bytes"] = int(g.get_option("hint_bytes"))
     log(f"store: {store} (budget {budget} B) in {time.perf_counter() - t0:.1f} s")
+    tuning = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuning = tuned(g.tune("bfs", sources=[args.source], partitions=True, repeats=3),
+                       time.perf_counter() - t0)
+        store = g.store_info()
+        store["hint_bytes"] = int(g.get_option("hint_bytes"))
+        log(f"tuned: {tuning}")
 
     for i in range(args.warmup):
         _, st = g.bfs(args.source, out=False)
@@ -277,8 +300,9 @@ def c2_block(T, O, args):
         "config": {"workload": workload,
                    "scale": args.scale, "edge_factor": args.edge_factor, "vertices": g.n,
                    "edges": g.m, "edge_bytes": g.edge_bytes, "budget_frac": args.budget,
-                   "partition_edges": args.partition_edges, "source": args.source,
-                   "store": store, "graph500_gteps": round(value / 2, 4)},
+                   "partition_edges": tuning["partition_edges"] if tuning else args.partition_edges,
+                   "source": args.source, "store": store, "tuning": tuning,
+                   "graph500_gteps": round(value / 2, 4)},
         "roofline": bfs_roofline(stats, workload),
     }
     lv, _ = g.bfs(args.source)  # levels for the CPU cross-check (untimed)
@@ -358,13 +382,19 @@ def sssp_block(T, O, args):
     deg = np.diff(g.offsets())
     rng = np.random.default_rng(1)
     src = int(rng.choice(np.nonzero(deg)[0]))
+    tuning = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuning = tuned(g.tune("sssp", sources=[src], partitions=True, repeats=2),
+                       time.perf_counter() - t0)
+        log(f"sssp tuned: {tuning}")
     g.sssp(src, out=False)
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.sssp_steps):
         stats.append(g.sssp(src, out=False)[1])
     out = {"workload": f"sssp-rmat{args.scale}-ef16-w8-budget30pct", "source": src,
-           "vertices": g.n, "edges": g.m, "store": g.store_info()}
+           "vertices": g.n, "edges": g.m, "store": g.store_info(), "tuning": tuning}
     out.update(step_summary(stats, time.perf_counter() - t0, "k_expand<SSSP>"))
     dist, _ = g.sssp(src)
     # algorithm-independent rate: edges out of the reached vertices per
@@ -399,11 +429,18 @@ def pr_block(T, O, args):
     for mode, transpose in (("pull", True), ("push", False)):
         g = T.Graph.rmat(args.scale, 16, seed=1, symmetric=False, transpose=transpose)
         g.store_config(int(0.25 * g.m * 4), args.partition_edges)
+        tuning = None
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            tuning = tuned(g.tune("pr", partitions=True, repeats=1, pr_iterations=5),
+                           time.perf_counter() - t0)
+            log(f"pr {mode} tuned: {tuning}")
         t0 = time.perf_counter()
         r, st = g.pr(mode)
         wall = time.perf_counter() - t0
         s = step_summary([st], wall, f"k_expand<PR_{mode.upper()}>")
         s["vertices"], s["edges"] = g.n, g.m
+        s["store"], s["tuning"] = g.store_info(), tuning
         out[mode] = s
         res[mode] = r
         if transpose and not args.no_check:
@@ -631,6 +668,8 @@ def main():
     ap.add_argument("--no-sssp", action="store_true")
     ap.add_argument("--no-pr", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes and partition size (no thv_tune)")
     ap.add_argument("--serve-scale", type=int, default=25)
     ap.add_argument("--serve-queries", type=int, default=20000)
     ap.add_argument("--workload", choices=["all", "bfs", "sssp", "pr", "serve"], default="all",

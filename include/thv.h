@@ -259,6 +259,49 @@ int thv_batch(thv_graph *g, int algo, const uint32_t *srcs, uint32_t k,
  * written to cache_path (nullable).  chosen receives 2 ints. */
 int thv_autotune(thv_graph *g, const char *cache_path, int *chosen);
 
+/* Tuning (no reference counterpart; it replaces the reference's fixed
+ * PARTITION_SIZE_MB, common.cuh:39, and k*numSMs x 1024 launch shapes,
+ * bfs/bfs.cu:21-27).  thv_tune times one workload on this graph under
+ * candidate configurations, one knob at a time (coordinate descent from
+ * the graph's current settings), and keeps the fastest:
+ *   expansion grid x edges per thread (2/4/8 blocks per CU x 4/8),
+ *   bottom-up probe grid (4/8/16 blocks per CU; BFS on symmetric graphs),
+ *   multi-source grid (4/8/16 blocks per CU; batches),
+ *   bottom-up full-scan degree (8/16; BFS on symmetric graphs),
+ *   SSSP bucket width (8/16/32/64),
+ *   partition size (2^14, 2^16, 2^18 edges; with partitions != 0, each
+ *   rebuilds the store under the same HBM budget).
+ * Every candidate's results are compared with the starting configuration's
+ * (bit-exact digests; PageRank within 1e-5 normalised L1): a difference is
+ * THV_ESTATE.  The choice stays on the handle and, keyed by device, graph
+ * (size, record bytes, options, sampled offsets), store budget and
+ * workload, is appended to cache_path (nullable); a later call with the same
+ * key applies the cached choice without timing. */
+#define THV_TUNE_BFS 0
+#define THV_TUNE_SSSP 1
+#define THV_TUNE_PR 2          /* push on a CSR graph, pull on a CSC graph */
+#define THV_TUNE_BATCH_BFS 3
+#define THV_TUNE_BATCH_SSSP 4
+typedef struct {
+  int workload;              /* THV_TUNE_*                                  */
+  uint32_t nsources;         /* timing sources, <= 64 (0: seeded defaults)  */
+  const uint32_t *sources;
+  int partitions;            /* 1: also try partition sizes (store rebuilds) */
+  int repeats;               /* timed runs per candidate (0: 3)             */
+  int pr_iterations;         /* PageRank iterations per timed run (0: 10)   */
+  const char *cache_path;    /* nullable                                    */
+} thv_tune_cfg;
+typedef struct {
+  int blocks_per_cu, edges_per_thread;  /* expansion shape                  */
+  int probe_blocks_per_cu, batch_blocks_per_cu;
+  uint32_t partition_edges, sssp_delta, pull_degree;
+  uint32_t candidates;       /* configurations timed (0: from the cache)     */
+  double default_ms;         /* median workload time, starting settings     */
+  double tuned_ms;           /*   and chosen settings                        */
+  int from_cache;
+} thv_tune_result;
+int thv_tune(thv_graph *g, const thv_tune_cfg *cfg, thv_tune_result *out);
+
 /* ---- serving (continuous batching) ------------------------------------ */
 
 typedef struct {

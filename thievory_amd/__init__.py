@@ -42,6 +42,23 @@ class ThvError(RuntimeError):
         self.code = code
 
 
+TUNE_WORKLOAD = {"bfs": 0, "sssp": 1, "pr": 2, "batch-bfs": 3, "batch-sssp": 4}
+
+
+class TuneCfg(C.Structure):
+    _fields_ = [("workload", C.c_int), ("nsources", C.c_uint32),
+                ("sources", C.c_void_p), ("partitions", C.c_int), ("repeats", C.c_int),
+                ("pr_iterations", C.c_int), ("cache_path", C.c_char_p)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int), ("edges_per_thread", C.c_int),
+                ("probe_blocks_per_cu", C.c_int), ("batch_blocks_per_cu", C.c_int),
+                ("partition_edges", C.c_uint32), ("sssp_delta", C.c_uint32),
+                ("pull_degree", C.c_uint32), ("candidates", C.c_uint32),
+                ("default_ms", C.c_double), ("tuned_ms", C.c_double), ("from_cache", C.c_int)]
+
+
 class Stats(C.Structure):
     _fields_ = [("edges_traversed", C.c_uint64), ("hbm_bytes", C.c_uint64),
                 ("host_bytes", C.c_uint64), ("moved_bytes", C.c_uint64),
@@ -113,6 +130,7 @@ def lib():
         "thv_batch": (C.c_int, [_vp, C.c_int, _vp, C.c_uint32, _vp, _vp, _vp, _vp,
                                 C.POINTER(Stats)]),
         "thv_autotune": (C.c_int, [_vp, C.c_char_p, _vp]),
+        "thv_tune": (C.c_int, [_vp, C.POINTER(TuneCfg), C.POINTER(TuneResult)]),
         "thv_serve_start": (C.c_int, [_vp, C.POINTER(ServeCfg), G]),
         "thv_serve_submit": (C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_uint64)]),
         "thv_serve_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(Result), _vp]),
@@ -302,6 +320,20 @@ class Graph:
         check(lib().thv_autotune(self._h, None if cache_path is None else os.fsencode(cache_path),
                                  _p(ch)))
         return {"blocks_per_cu": int(ch[0]), "edges_per_thread": int(ch[1])}
+
+    def tune(self, workload="bfs", sources=None, partitions=False, repeats=3, pr_iterations=10,
+             cache_path=None):
+        """thv_tune: times `workload` under candidate settings (launch shapes,
+        probe and batch grids, full-scan degree, SSSP bucket width and, with
+        partitions, partition sizes), keeps the fastest on the handle and
+        returns the choice with the starting and tuned median times."""
+        src = None if sources is None else np.ascontiguousarray(sources, np.uint32)
+        cfg = TuneCfg(TUNE_WORKLOAD[workload], 0 if src is None else len(src),
+                      None if src is None else src.ctypes.data, int(partitions), repeats,
+                      pr_iterations, None if cache_path is None else os.fsencode(cache_path))
+        r = TuneResult()
+        check(lib().thv_tune(self._h, C.byref(cfg), C.byref(r)))
+        return {f: getattr(r, f) for f, _ in TuneResult._fields_}
 
 
 class Server:

@@ -281,6 +281,17 @@ int thv_batch(thv_graph *gh, int algo, const uint32_t *srcs, uint32_t k, uint64_
   });
 }
 
+int thv_tune(thv_graph *gh, const thv_tune_cfg *cfg, thv_tune_result *out) {
+  return guarded([&] {
+    NEED(gh);
+    if (!cfg) fail(THV_EINVAL, "null tuning configuration");
+    Graph *g = reinterpret_cast<Graph *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (g->servers.load()) fail(THV_EBUSY, "a server is running on this graph");
+    tune(*g, *cfg, out);
+  });
+}
+
 int thv_autotune(thv_graph *gh, const char *cache_path, int *chosen) {
   return guarded([&] {
     NEED(gh);

@@ -810,7 +810,7 @@ void MultiEngine::launch_level() {
       if (level_bu) {
         launch_bottom_up(active & ~td);
         if (td) {
-          hipLaunchKernelGGL(k_ms_lanebits, dim3(grid_for(n, 8 * g.cus)), dim3(MBS), 0, stream,
+          hipLaunchKernelGGL(k_ms_lanebits, dim3(grid_for(n, g.shape.batch_bpc * g.cus)), dim3(MBS), 0, stream,
                              F[cur].as<unsigned long long>(), n, td, tbits.as<uint32_t>());
           launch_compact(g, qs, tbits.as<uint32_t>(), c, stream);
           launch_top_down(c, td);
@@ -861,7 +861,7 @@ void MultiEngine::launch_bottom_up(uint64_t lanes) {
     }
   }
   if (qs.slist.bytes < n * 4) fail(THV_ESTATE, "bottom-up queue too small");
-  const int gs = grid_for(n, 8 * g.cus), gl = 8 * g.cus;
+  const int gs = grid_for(n, g.shape.batch_bpc * g.cus), gl = g.shape.batch_bpc * g.cus;
   if (g.rec == 8) {
     hipLaunchKernelGGL(k_ms_bu<8>, dim3(gs), dim3(MBS), 0, stream, b);
     hipLaunchKernelGGL(k_ms_bu_long<8>, dim3(gl), dim3(MBS), 0, stream, b);

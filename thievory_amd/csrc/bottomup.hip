@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *k
 }  // namespace
 
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
-  const int grid = 8 * g.cus;
+  const int grid = g.shape.probe_bpc * g.cus;
   if (g.rec == 8) hipLaunchKernelGGL(k_bottom_up<8>, dim3(grid), dim3(BBS), 0, s, a);
   else hipLaunchKernelGGL(k_bottom_up<4>, dim3(grid), dim3(BBS), 0, s, a);
   HIPC(hipGetLastError());

@@ -957,7 +957,7 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   p.ccin = ccin;
   p.cout = cout;
   p.ccout = ccout;
-  const uint32_t grid = std::min<uint32_t>(p.ntiles, 8u * (uint32_t)g.cus);
+  const uint32_t grid = std::min<uint32_t>(p.ntiles, (uint32_t)(g.shape.probe_bpc * g.cus));
   if (t0) HIPC(hipEventRecord(t0, s));
   if (cin) {
     const uint32_t groups = (uint32_t)div_up((uint64_t)p.ntiles * (CBS / 64), LR);

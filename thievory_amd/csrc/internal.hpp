@@ -167,6 +167,9 @@ struct Workspace {
 struct Shape {
   int blocks_per_cu = 4;      // expansion grid = blocks_per_cu * CUs
   int ept = 8;                // edges per thread per chunk (chunk = 256*ept)
+  int probe_bpc = 8;          // bottom-up probe, short-list and early-exit
+                              // grids = probe_bpc * CUs (persistent)
+  int batch_bpc = 8;          // multi-source bottom-up grids = batch_bpc * CUs
 };
 
 // ---- graph -------------------------------------------------------------------
@@ -338,6 +341,7 @@ void run_batch(Graph &g, int algo, const uint32_t *srcs, uint32_t k,
 
 // autotune.cpp
 void autotune(Graph &g, const char *cache_path, int *chosen);
+void tune(Graph &g, const thv_tune_cfg &cfg, thv_tune_result *out);
 
 // shared hash (identical to the oracle's): SplitMix64 output function
 __host__ __device__ inline uint64_t mix64(uint64_t z) {
