@@ -241,6 +241,41 @@ int thv_sssp(thv_graph *g, uint32_t src, uint32_t *out, thv_stats *st);
 int thv_pr(thv_graph *g, int mode, float damping, float tol, int max_iter,
            float *out, thv_stats *st);
 
+/* ---- 64-bit vertex ids ------------------------------------------------- */
+
+/* Graphs of 2^32 - 1 vertices or more: the reference declares BFS64, CC64
+ * and SSSP64 (bfs/bfs.cu:544, cc/cc.cu:616, sssp/sssp.cu:572) but leaves
+ * them empty, and its file format holds 4-byte destinations, so these
+ * graphs enter through arrays.  The edge lists are given per listed vertex
+ * (a vertex with edges): k strictly ascending ids < n, k + 1 list offsets
+ * from 0 (every list non-empty) and loff[k] destinations < n, with
+ * optional u32 weights (needed by SSSP).  Values are n x u64 with the
+ * conventions above (unreached / none: n + 1).  The store is split as for
+ * thv_store_config (greedy runs of lists of at most partition_edges
+ * records, densest in HBM up to the budget, the rest zero-copy); the
+ * default is all in HBM with 2^16-record runs.  Traversal is top-down.
+ * digest3 (nullable) receives the sum of mix64(mix64(v) + value) over the
+ * vertices with a value other than n + 1 (mod 2^64; CC: over all), their
+ * count and the largest value; out (nullable) the n values. */
+typedef struct thv_graph64 thv_graph64;
+int thv_graph64_from_lists(uint64_t n, uint64_t k, const uint64_t *ids,
+                           const uint64_t *loff, const uint64_t *dst,
+                           const uint32_t *w, thv_graph64 **out);
+int thv_graph64_store_config(thv_graph64 *g, uint64_t device_edge_budget_bytes,
+                             uint32_t partition_edges);
+int thv_graph64_info(const thv_graph64 *g, uint64_t *n, uint64_t *m,
+                     uint32_t *partitions, uint32_t *device_partitions,
+                     uint64_t *device_edges);
+int thv_bfs64(thv_graph64 *g, uint64_t src, uint64_t *out, thv_stats *st,
+              uint64_t *digest3);
+int thv_cc64(thv_graph64 *g, uint64_t *out, thv_stats *st, uint64_t *digest3);
+int thv_sssp64(thv_graph64 *g, uint64_t src, uint64_t *out, thv_stats *st,
+               uint64_t *digest3);
+/* The last algorithm call's values at k given vertices. */
+int thv_graph64_values(thv_graph64 *g, const uint64_t *vertices, uint64_t k,
+                       uint64_t *out);
+void thv_graph64_close(thv_graph64 *g);
+
 /* ---- batched multi-source queries (no reference counterpart) ---------- */
 
 /* Runs k <= 64 BFS (algo THV_ALGO_BFS) or SSSP (THV_ALGO_SSSP) queries in

@@ -131,6 +131,15 @@ def lib():
                                 C.POINTER(Stats)]),
         "thv_autotune": (C.c_int, [_vp, C.c_char_p, _vp]),
         "thv_tune": (C.c_int, [_vp, C.POINTER(TuneCfg), C.POINTER(TuneResult)]),
+        "thv_graph64_from_lists": (C.c_int, [C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp,
+                                             C.POINTER(_vp)]),
+        "thv_graph64_store_config": (C.c_int, [_vp, C.c_uint64, C.c_uint32]),
+        "thv_graph64_info": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "thv_bfs64": (C.c_int, [_vp, C.c_uint64, _vp, C.POINTER(Stats), _vp]),
+        "thv_cc64": (C.c_int, [_vp, _vp, C.POINTER(Stats), _vp]),
+        "thv_sssp64": (C.c_int, [_vp, C.c_uint64, _vp, C.POINTER(Stats), _vp]),
+        "thv_graph64_values": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
+        "thv_graph64_close": (None, [_vp]),
         "thv_serve_start": (C.c_int, [_vp, C.POINTER(ServeCfg), G]),
         "thv_serve_submit": (C.c_int, [_vp, C.c_int, C.c_uint32, C.POINTER(C.c_uint64)]),
         "thv_serve_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(Result), _vp]),
@@ -334,6 +343,69 @@ class Graph:
         r = TuneResult()
         check(lib().thv_tune(self._h, C.byref(cfg), C.byref(r)))
         return {f: getattr(r, f) for f, _ in TuneResult._fields_}
+
+
+class Graph64:
+    """A graph with 64-bit vertex ids (thv_graph64_*): the sorted ids of the
+    vertices with edges, their list offsets and u64 destinations (optional
+    u32 weights); values are n x u64."""
+
+    def __init__(self, n, ids, loff, dst, w=None):
+        self._h = None
+        self.n = int(n)
+        ids = np.ascontiguousarray(ids, np.uint64)
+        loff = np.ascontiguousarray(loff, np.uint64)
+        dst = np.ascontiguousarray(dst, np.uint64)
+        w = None if w is None else np.ascontiguousarray(w, np.uint32)
+        h = _vp()
+        check(lib().thv_graph64_from_lists(self.n, len(ids), _p(ids), _p(loff), _p(dst),
+                                           _p(w), C.byref(h)))
+        self._h = h
+        self.m = int(loff[-1]) if len(loff) else 0
+
+    def store_config(self, budget_bytes=None, partition_edges=1 << 16):
+        check(lib().thv_graph64_store_config(
+            self._h, 2 ** 64 - 1 if budget_bytes is None else int(budget_bytes), partition_edges))
+
+    def store_info(self):
+        p, pd, de = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().thv_graph64_info(self._h, None, None, C.byref(p), C.byref(pd), C.byref(de)))
+        return {"partitions": p.value, "device_partitions": pd.value,
+                "device_edges": de.value}
+
+    def _run(self, fn, *args, out=False):
+        o = np.zeros(self.n, np.uint64) if out else None
+        st = Stats()
+        dg = np.zeros(3, np.uint64)
+        check(fn(self._h, *args, _p(o), C.byref(st), _p(dg)))
+        return {"values": o, "hash": int(dg[0]), "reached": int(dg[1]),
+                "depth": int(dg[2]), "stats": st}
+
+    def bfs(self, src, out=False):
+        return self._run(lib().thv_bfs64, int(src), out=out)
+
+    def cc(self, out=False):
+        return self._run(lib().thv_cc64, out=out)
+
+    def sssp(self, src, out=False):
+        return self._run(lib().thv_sssp64, int(src), out=out)
+
+    def values(self, vertices):
+        at = np.ascontiguousarray(vertices, np.uint64)
+        o = np.zeros(len(at), np.uint64)
+        check(lib().thv_graph64_values(self._h, _p(at), len(at), _p(o)))
+        return o
+
+    def close(self):
+        if self._h:
+            lib().thv_graph64_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

@@ -4,6 +4,7 @@
 #include <new>
 
 #include "internal.hpp"
+#include "wide.hpp"
 
 namespace thv {
 void set_last_error(const std::string &m);
@@ -279,6 +280,76 @@ int thv_batch(thv_graph *gh, int algo, const uint32_t *srcs, uint32_t k, uint64_
     std::lock_guard<std::mutex> lk(g->mu);
     run_batch(*g, algo, srcs, k, out_hash, out_reached, out_depth, out_full, st);
   });
+}
+
+int thv_graph64_from_lists(uint64_t n, uint64_t k, const uint64_t *ids, const uint64_t *loff,
+                           const uint64_t *dst, const uint32_t *w, thv_graph64 **out) {
+  return guarded([&] {
+    NEED(out);
+    *out = reinterpret_cast<thv_graph64 *>(graph64_from_lists(n, k, ids, loff, dst, w));
+  });
+}
+
+int thv_graph64_store_config(thv_graph64 *gh, uint64_t budget, uint32_t partition_edges) {
+  return guarded([&] {
+    NEED(gh);
+    Graph64 *g = reinterpret_cast<Graph64 *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    graph64_store(*g, budget, partition_edges);
+  });
+}
+
+int thv_graph64_info(const thv_graph64 *gh, uint64_t *n, uint64_t *m, uint32_t *partitions,
+                     uint32_t *device_partitions, uint64_t *device_edges) {
+  return guarded([&] {
+    NEED(gh);
+    const Graph64 *g = reinterpret_cast<const Graph64 *>(gh);
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (partitions) *partitions = g->P;
+    if (device_partitions) *device_partitions = g->Pdev;
+    if (device_edges) *device_edges = g->dev_edges;
+  });
+}
+
+static int wide_call(thv_graph64 *gh, int algo, uint64_t src, uint64_t *out, thv_stats *st,
+                     uint64_t *digest3) {
+  return guarded([&] {
+    NEED(gh);
+    Graph64 *g = reinterpret_cast<Graph64 *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    run_wide(*g, algo, src, out, st, digest3);
+  });
+}
+
+int thv_bfs64(thv_graph64 *g, uint64_t src, uint64_t *out, thv_stats *st, uint64_t *digest3) {
+  return wide_call(g, THV_ALGO_BFS, src, out, st, digest3);
+}
+
+int thv_cc64(thv_graph64 *g, uint64_t *out, thv_stats *st, uint64_t *digest3) {
+  return wide_call(g, THV_ALGO_CC, 0, out, st, digest3);
+}
+
+int thv_sssp64(thv_graph64 *g, uint64_t src, uint64_t *out, thv_stats *st, uint64_t *digest3) {
+  return wide_call(g, THV_ALGO_SSSP, src, out, st, digest3);
+}
+
+int thv_graph64_values(thv_graph64 *gh, const uint64_t *vertices, uint64_t k, uint64_t *out) {
+  return guarded([&] {
+    NEED(gh);
+    if (k && (!vertices || !out)) fail(THV_EINVAL, "null argument");
+    Graph64 *g = reinterpret_cast<Graph64 *>(gh);
+    std::lock_guard<std::mutex> lk(g->mu);
+    wide_gather(*g, vertices, k, out);
+  });
+}
+
+void thv_graph64_close(thv_graph64 *gh) {
+  if (!gh) return;
+  try {
+    delete reinterpret_cast<Graph64 *>(gh);
+  } catch (...) {
+  }
 }
 
 int thv_tune(thv_graph *gh, const thv_tune_cfg *cfg, thv_tune_result *out) {
