@@ -687,6 +687,9 @@ def main():
     dev = T.device_info()
     log(f"device {dev}")
     t_start = time.perf_counter()
+    # the measured link and memory ceilings (thv_link_probe, 1 GiB buffers)
+    ceil = T.link_probe(1 << 30)
+    log(f"ceilings {ceil}")
     if args.workload == "sssp":
         print(json.dumps(sssp_block(T, O, args)), flush=True)
         return
@@ -697,12 +700,22 @@ def main():
         print(json.dumps({"serving": serving_block(T, O, args)}), flush=True)
         return
     out = c2_block(T, O, args)
+    out["ceilings"] = ceil
+    zc = ceil["zero_copy_read_GBps"]
+    if "c2_topdown" in out:
+        k = out["c2_topdown"]["kernel"]
+        k["frac_of_measured_zero_copy"] = round(k["achieved_GBps"] / zc, 4)
     if args.workload == "all":
         if not args.no_sssp:
             out["sssp"] = sssp_block(T, O, args)
+            r = out["sssp"]["roofline"]
+            r["frac_of_measured_zero_copy"] = round(r["achieved"] / zc, 4)
             log(f"sssp: {out['sssp']}")
         if not args.no_pr:
             out["pr"] = pr_block(T, O, args)
+            for m in ("push", "pull"):
+                r = out["pr"][m]["roofline"]
+                r["frac_of_measured_zero_copy"] = round(r["achieved"] / zc, 4)
             log(f"pr: {out['pr']}")
         if not args.no_serve:
             out["serving"] = serving_block(T, O, args)

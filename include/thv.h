@@ -381,6 +381,13 @@ const char *thv_last_error(void);
 const char *thv_version(void);
 int thv_device_info(char *name, size_t name_len, int *cus,
                     uint64_t *hbm_bytes);
+/* Ceilings of the current device (no reference counterpart), over buffers
+ * of `bytes` bytes, best of three timed runs: out[0] a streaming kernel's
+ * zero-copy read of mapped pinned host memory (the host share of the edge
+ * store), out[1] pinned host -> device DMA, out[2] the same kernel's read of
+ * HBM, out[3] a device -> device copy (bytes read + written); all GB/s. */
+int thv_link_probe(uint64_t bytes, double *out4);
+
 /* Free and total device memory (hipMemGetInfo). */
 int thv_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
 

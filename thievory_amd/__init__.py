@@ -131,6 +131,7 @@ def lib():
                                 C.POINTER(Stats)]),
         "thv_autotune": (C.c_int, [_vp, C.c_char_p, _vp]),
         "thv_tune": (C.c_int, [_vp, C.POINTER(TuneCfg), C.POINTER(TuneResult)]),
+        "thv_link_probe": (C.c_int, [C.c_uint64, _vp]),
         "thv_graph64_from_lists": (C.c_int, [C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp,
                                              C.POINTER(_vp)]),
         "thv_graph64_store_config": (C.c_int, [_vp, C.c_uint64, C.c_uint32]),
@@ -176,6 +177,16 @@ def _out(out, n, dtype):
     if out is False:
         return None
     return np.zeros(n, dtype) if out is None else out
+
+
+def link_probe(nbytes=1 << 30):
+    """thv_link_probe: GB/s of a zero-copy kernel read of mapped pinned host
+    memory, pinned host -> device DMA, a kernel read of HBM and a device
+    copy (read + written), over buffers of nbytes."""
+    o = np.zeros(4, np.float64)
+    check(lib().thv_link_probe(int(nbytes), _p(o)))
+    return {"zero_copy_read_GBps": float(o[0]), "dma_h2d_GBps": float(o[1]),
+            "hbm_read_GBps": float(o[2]), "hbm_copy_GBps": float(o[3]), "bytes": int(nbytes)}
 
 
 def device_info():

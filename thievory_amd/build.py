@@ -33,7 +33,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
 
 LIB_SOURCES = ["graph.cpp", "io.cpp", "abi.cpp", "frontier.hip", "expand.hip",
                "algos.hip", "rmat.hip", "batch.hip", "serve.cpp", "autotune.cpp",
-               "bottomup.hip", "cc.hip", "noalloc.hip", "wide.hip"]
+               "bottomup.hip", "cc.hip", "noalloc.hip", "wide.hip", "probe.hip"]
 CLI_SOURCES = ["cli.cpp"]
 # stand-alone host tools (no GPU): format converter and values.bin comparer
 TOOLS = {"thv_convert": "convert.cpp", "thv_compare": "compare.cpp"}

@@ -10,6 +10,7 @@ namespace thv {
 void set_last_error(const std::string &m);
 const char *last_error();
 Graph *graph_open(const char *path, int fmt, int edge_bytes);
+void link_probe(uint64_t bytes, double out[4]);
 Graph *graph_from_arrays(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *dst,
                          const uint32_t *w, const uint32_t *outdeg, int edge_bytes);
 void graph_export(const Graph &g, uint64_t *off, uint32_t *dst, uint32_t *w, uint32_t *outdeg);
@@ -350,6 +351,13 @@ void thv_graph64_close(thv_graph64 *gh) {
     delete reinterpret_cast<Graph64 *>(gh);
   } catch (...) {
   }
+}
+
+int thv_link_probe(uint64_t bytes, double *out4) {
+  return guarded([&] {
+    NEED(out4);
+    link_probe(bytes, out4);
+  });
 }
 
 int thv_tune(thv_graph *gh, const thv_tune_cfg *cfg, thv_tune_result *out) {
