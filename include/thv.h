@@ -168,6 +168,11 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       2^20, >= 1).  A link that needs more steps raises
  *                       a flag and CC reruns by label propagation; the
  *                       labels are the same either way.
+ *   THV_OPT_KERNEL_TIMING 1 (default): BFS brackets its expansion, probe
+ *                       and bottom-up launches with HIP events and reports
+ *                       their time (thv_stats expand_ms, probe_ms,
+ *                       bottom_up_ms); 0: no events, those stay 0 (each
+ *                       event is a few microseconds of queue time).
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -181,6 +186,7 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_SSSP_DELTA 10
 #define THV_OPT_CC_UNION 11
 #define THV_OPT_CC_UF_STEPS 12
+#define THV_OPT_KERNEL_TIMING 13
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

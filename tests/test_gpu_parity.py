@@ -161,6 +161,36 @@ def test_bfs_direction_optimizing(rmat16, frac):
     g.set_option("do_alpha", 14.0)
 
 
+@pytest.mark.parametrize("frac", [1.0, 0.25])
+def test_bfs_kernel_timing_off(rmat16, frac):
+    """Without HIP events (kernel_timing 0) a direction-optimising BFS gives
+    the levels and every count of a timed run; only the kernel times are 0.
+    Its top-down levels' host partitions come from the counters the next
+    level's count pass publishes: a split store reports some."""
+    g, og = rmat16
+    _say("case: bfs without kernel timing", frac)
+    g.store_config(int(frac * g.m * g.edge_bytes), 4096)
+    times = ("ms", "expand_ms", "probe_ms", "bottom_up_ms")
+    try:
+        for s in (1, 12345):
+            ref = O.bfs(og, s)
+            g.set_option("kernel_timing", 1)
+            lv1, st1 = g.bfs(s)
+            g.set_option("kernel_timing", 0)
+            assert g.get_option("kernel_timing") == 0.0
+            lv0, st0 = g.bfs(s)
+            assert np.array_equal(lv1, ref) and np.array_equal(lv0, ref), s
+            d1, d0 = st1.as_dict(), st0.as_dict()
+            assert {k: d0[k] for k in d0 if k not in times} == \
+                {k: d1[k] for k in d1 if k not in times}, s
+            assert st0.expand_ms == st0.probe_ms == st0.bottom_up_ms == 0.0
+            assert st1.expand_ms > 0 and st1.bottom_up_levels > 0 and st1.bottom_up_ms > 0
+            if frac < 1.0:
+                assert st1.host_partitions_touched > 0, s
+    finally:
+        g.set_option("kernel_timing", 1)
+
+
 def _beads(k, s):
     """k cliques of s vertices in a row, clique b's last vertex joined to
     clique b+1's first: from vertex 0 the frontier alternates between one

@@ -135,6 +135,7 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
         NEED(value >= 0.0 && value <= 16.0);
         g->do_pull_degree = (uint32_t)value;
         break;
+      case THV_OPT_KERNEL_TIMING: g->kernel_timing = value != 0.0; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });
@@ -157,6 +158,7 @@ int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
       case THV_OPT_SSSP_DELTA: *value = g->sssp_delta; break;
       case THV_OPT_CC_UNION: *value = g->cc_union ? 1.0 : 0.0; break;
       case THV_OPT_CC_UF_STEPS: *value = g->uf_max_steps; break;
+      case THV_OPT_KERNEL_TIMING: *value = g->kernel_timing ? 1.0 : 0.0; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });

@@ -41,6 +41,42 @@ __global__ void k_bits_prefix(uint32_t *bm, uint64_t n, uint64_t words) {
 
 __global__ void k_set_bit(uint32_t *bm, uint32_t v) { bm[v >> 5] |= 1u << (v & 31); }
 
+// The start of a BFS from src in one launch: every value n + 1 but src's 1
+// (16-byte stores over the 256-byte aligned array); visited = the degree-0
+// vertices (iso, may be null) and src; frontier f0 = {src}; f1 clear; the
+// u64 counters z0[0, nz0) and z1[0, nz1) zero.
+__global__ void __launch_bounds__(256)
+k_bfs_init(uint32_t *val, uint64_t n, uint32_t src, uint32_t *vis, const uint32_t *iso,
+           uint32_t *f0, uint32_t *f1, uint64_t words, unsigned long long *z0, uint32_t nz0,
+           unsigned long long *z1, uint32_t nz1) {
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+  const uint32_t far = (uint32_t)(n + 1);
+  uint4 *v4 = reinterpret_cast<uint4 *>(val);
+  for (uint64_t i = t; i < n / 4; i += T) {
+    uint4 x = make_uint4(far, far, far, far);
+    if (i == src / 4) {
+      const uint32_t k = src & 3u;
+      if (k == 0) x.x = 1u;
+      else if (k == 1) x.y = 1u;
+      else if (k == 2) x.z = 1u;
+      else x.w = 1u;
+    }
+    v4[i] = x;
+  }
+  for (uint64_t i = n / 4 * 4 + t; i < n; i += T) val[i] = i == src ? 1u : far;
+  const uint64_t sw = src >> 5;
+  const uint32_t sb = 1u << (src & 31);
+  for (uint64_t i = t; i < words; i += T) {
+    const uint32_t b = i == sw ? sb : 0u;
+    vis[i] = (iso ? iso[i] : 0u) | b;
+    f0[i] = b;
+    f1[i] = 0u;
+  }
+  if (t < nz0) z0[t] = 0ull;
+  if (t < nz1) z1[t] = 0ull;
+}
+
 static int fill_grid(uint64_t n) {
   uint64_t b = (n + 255) / 256;
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(b, 8192));
@@ -195,7 +231,7 @@ static ExpandArgs base_args(Graph &g) { return base_args(g, g.ws); }
 static void value_loop(Graph &g, Workspace &w, Algo algo, thv_stats *st) {
   hipStream_t s = g.stream;
   const uint64_t words = g.n / 32 + 2;
-  const bool timed = w.ev[0] && w.ev[1];
+  const bool timed = w.ev[0] && w.ev[1] && g.kernel_timing;
   ExpandArgs a = base_args(g, w);
   LevelCtr *d_ctr = w.qs.ctr.as<LevelCtr>();
   int cur = 0;
@@ -271,6 +307,29 @@ static void check_src(Graph &g, uint32_t src) {
 // ones, which stops at the first frontier neighbour.  edges_traversed keeps
 // the algorithm-independent count (sum of degrees of reached vertices);
 // edges_examined counts the records actually read.
+//
+// Host round trips: one stream sync per level (two when a probe's counts
+// decide the direction), and no memset or readback copy.  The kernel that
+// finishes a level's counters writes them to the host-mapped readback (the
+// count pass's scan, the probe's last launch, a one-wave launch after the
+// early-exit kernel); the count pass or the probe clears the next level's
+// frontier bitmap, one of three in rotation.
+constexpr int AUX_PC = 4;      // aux[4, 10): probe counters
+constexpr int HAUX_PC = 8;     // h_aux[8, 14): published probe counters
+
+// Elapsed time between two recorded events that the stream has passed
+// (the runtime may not have marked them complete yet).
+static float event_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  hipError_t e = hipEventElapsedTime(&ms, a, b);
+  if (e == hipErrorNotReady) {
+    HIPC(hipEventSynchronize(b));
+    e = hipEventElapsedTime(&ms, a, b);
+  }
+  HIPC(e);
+  return ms;
+}
+
 static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   Workspace &w = g.ws;
   hipStream_t s = g.stream;
@@ -280,15 +339,16 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   LevelCtr *c_long = d_ctr + 2, *c_short = d_ctr + 3;
   const uint64_t rec = (uint64_t)g.rec;
   const uint32_t D = g.do_pull_degree;
-  int cur = 0;
+  int cur = 0;  // this level's frontier: front[cur]; next: front[(cur + 1) % 3]
   bool bu = false;
+  const bool timed = g.kernel_timing;
   const bool trace = getenv("THV_TRACE") != nullptr;
   const bool host_share = g.store.host_vertices > 0;
   const double link = g.do_link_gbps * 1e9;
   const bool hints = g.store.d_hint.p != nullptr;
   // probe counters: hint hits, unvisited, unvisited with a hint, words
   // written, candidates written, candidates read
-  unsigned long long *d_pc = w.aux.as<unsigned long long>() + 4;
+  unsigned long long *d_pc = w.aux.as<unsigned long long>() + AUX_PC;
   // Cost terms of the split-store rule, measured on MI355X at R-MAT 26:
   // a short host list scanned in full costs its bytes plus SHORT_NS (a
   // partly shared request), a long one do_vertex_ns (one early-exit read);
@@ -319,7 +379,6 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0, last_probe_ms = 0;
   uint32_t launches = 0, probes = 0;
   uint64_t probe_bytes = 0;  // the parts that do not come from d_pc
-  HIPC(hipMemsetAsync(d_pc, 0, 6 * 8, s));
   // a bottom-up level writes the values of the vertices it places and sums
   // the next frontier (c_next), so the next level needs no count pass
   LevelCtr *c_next = d_ctr + 4;
@@ -329,15 +388,28 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   // later probe reads only those (cand: the buffer holding the last list)
   int cand = -1;
   const uint64_t ntiles = (g.n + TILE - 1) / TILE;
-  auto probe = [&](const uint32_t *hint, uint32_t *next, uint32_t level_value) {
-    HIPC(hipMemsetAsync(c_next, 0, sizeof(LevelCtr), s));
+  // the probe (c_next starts at zero: the count pass or the last early-exit
+  // kernel reset it) publishes the long, short and next-frontier counters
+  // (slots 2-4) and the probe counters; clear: the bitmap to clear, when no
+  // count pass of this level did
+  auto probe = [&](const uint32_t *front, const uint32_t *hint, uint32_t *next,
+                   uint32_t level_value, uint32_t *clear) {
     const int out = cand < 0 ? 0 : cand ^ 1;
     QueueSet &q = w.qs;
-    launch_bu_probe(g, q, w.front[cur].as<uint32_t>(), w.vis.as<uint32_t>(), next, hint, D,
-                    c_short, c_long, d_pc, s, w.ev[2], w.ev[3], c_next, w.val.as<uint32_t>(),
-                    level_value, cand < 0 ? nullptr : q.clist[cand].as<uint32_t>(),
+    LevelSide ps;
+    ps.zero = clear;
+    ps.nzero = clear ? words : 0;
+    ps.pub_src = c_long;
+    ps.pub = w.h_ctr_dev + 2;
+    ps.npub = 3;
+    ps.xsrc = d_pc;
+    ps.xpub = w.h_aux_dev + HAUX_PC;
+    ps.nx = 6;
+    launch_bu_probe(g, q, front, w.vis.as<uint32_t>(), next, hint, D, c_short, c_long, d_pc, s,
+                    timed ? w.ev[2] : nullptr, timed ? w.ev[3] : nullptr, c_next, w.val.as<uint32_t>(), level_value,
+                    cand < 0 ? nullptr : q.clist[cand].as<uint32_t>(),
                     cand < 0 ? nullptr : q.ccnt[cand].as<uint32_t>(), q.clist[out].as<uint32_t>(),
-                    q.ccnt[out].as<uint32_t>());
+                    q.ccnt[out].as<uint32_t>(), &ps);
     // a sweep reads the visited and residency words and writes the tile
     // tables of both sets, a list probe reads its regions' candidate
     // counts (the candidates themselves are counted on the device); both
@@ -347,8 +419,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     cand = out;
   };
   auto probe_done = [&]() {  // after the stream has drained past the probe
-    float pm = 0;
-    HIPC(hipEventElapsedTime(&pm, w.ev[2], w.ev[3]));
+    const float pm = timed ? event_ms(w.ev[2], w.ev[3]) : 0.0f;
     probe_ms += pm;
     last_probe_ms = pm;
     // short lists: an address each, their HBM records, and a frontier word
@@ -379,8 +450,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     if (p.bu && (!p.probed || p.sure)) probe_done();
     if (p.sure) p.bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
     prev_bu_cost = p.bu && p.probed ? p.bu_cost : -1;
-    float ms = 0;
-    HIPC(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+    const float ms = timed ? event_ms(w.ev[0], w.ev[1]) : 0.0f;
     LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
     memset(&hs, 0, sizeof hs);
     memset(&hb, 0, sizeof hb);
@@ -399,7 +469,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
               "long_host=%llu read_host=%llu read_hbm=%llu kernel_ms=%.3f probe_ms=%.3f\n",
               p.L, p.bu ? "bottom-up" : "top-down", (unsigned long long)p.nf,
               (unsigned long long)p.mf, p.h.qw[0], p.td_cost * 1e3, p.bu_cost * 1e3,
-              p.hinted ? w.h_aux[1] : 0ull,
+              p.hinted ? w.h_aux[HAUX_PC] : 0ull,
               hs.qn[0] + hs.qn[1], hs.qw[0], hb.qn[0] + hb.qn[1], hb.qn[0],
               p.bu ? hs.qw[0] + hb.rd[0] : p.h.qw[0], p.bu ? hs.qw[1] + hb.rd[1] : p.h.qw[1], ms,
               p.bu || p.probed ? last_probe_ms : 0.0);
@@ -413,7 +483,9 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         st->edges_examined += p.mf;
         st->host_bytes += p.h.qw[0] * rec;
         st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + 8 * words;
-        st->host_partitions_touched += p.h.touched[0];
+        // k_touched added them after the count this level read; the next
+        // level's count pass published them with its own totals
+        st->host_partitions_touched += w.h_ctr[p.L & 1].touched[0];
       } else {
         const uint64_t rh = hs.qw[0] + hb.rd[0], rd = hs.qw[1] + hb.rd[1];
         st->edges_examined += rh + rd;
@@ -432,20 +504,32 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   const uint64_t t0 = now_ns();
   for (uint32_t L = 0;; L++) {
     LevelCtr *c = d_ctr + (L & 1);
+    uint32_t *front = w.front[cur].as<uint32_t>();
+    uint32_t *next = w.front[(cur + 1) % 3].as<uint32_t>();
+    uint32_t *dead = w.front[(cur + 2) % 3].as<uint32_t>();  // the last level's frontier
     // fused: the last level ran bottom-up, wrote this frontier's values and
-    // summed it; otherwise count it (and write its values) now
+    // summed it (its early-exit kernel published the sums in slot 4);
+    // otherwise count it (and write its values) now.  The count pass
+    // publishes slots 0-1 (this level's totals and the partitions the last
+    // level's write pass touched), resets c_next and clears `dead` for the
+    // next level; a fused level has its probe or compaction clear it.
     const bool fused = fused_next;
-    if (fused) {
-      HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c_next, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
-    } else {
-      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s, CMP_COUNT_ONLY,
-                     w.val.as<uint32_t>(), L + 1);
-      HIPC(hipMemcpyAsync(&w.h_ctr[L & 1], c, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
+    if (!fused) {
+      LevelSide cs;
+      cs.zero = dead;
+      cs.nzero = words;
+      cs.pub_src = d_ctr;
+      cs.pub = w.h_ctr_dev;
+      cs.npub = 2;
+      cs.zctr = c_next;
+      launch_compact(g, w.qs, front, c, s, CMP_COUNT_ONLY, w.val.as<uint32_t>(), L + 1, 1,
+                     0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &cs);
     }
     HIPC(hipStreamSynchronize(s));
     finish_level(pend);
     pend.valid = false;
-    const LevelCtr h = w.h_ctr[L & 1];
+    const LevelCtr h = fused ? w.h_ctr[4] : w.h_ctr[L & 1];
+    uint32_t *clear = fused ? dead : nullptr;
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
     if (fused) last_cand = h.cand;
     if (nf == 0) break;
@@ -455,9 +539,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     // unvisited lists, so a top-down level needs a fresh frontier count)
     bool probed = false, hinted = false, sure = false;
     double td_cost = 0, bu_cost = 0;
-    HIPC(hipMemsetAsync(w.front[cur ^ 1].p, 0, words * 4, s));
     const uint32_t *hint = hints ? g.store.d_hint.as<uint32_t>() : nullptr;
-    uint32_t *next = w.front[cur ^ 1].as<uint32_t>();
     if (host_share) {
       // top-down streams the frontier's host records over the link and
       // writes and expands its queue in HBM; the hybrid bottom-up level
@@ -470,7 +552,8 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
           cand < 0 ? (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED : list_cost();
       bu = false;
       if (td_cost > probe_cost) {
-        probe(hint, next, L + 2);
+        probe(front, hint, next, L + 2, clear);
+        clear = nullptr;
         probed = true;
         hinted = hints;
         // the short and long unvisited host lists of this level are subsets
@@ -481,12 +564,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         if (prev_bu_cost >= 0 && prev_bu_cost < td_cost) {
           bu = sure = true;
         } else {
-          HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost,
-                              s));
-          HIPC(hipMemcpyAsync(&w.h_ctr[4], c_next, sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
-          if (trace && hints)
-            HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
-          HIPC(hipStreamSynchronize(s));
+          HIPC(hipStreamSynchronize(s));  // the probe published slots 2-4
           probe_done();
           bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
           bu = bu_cost < td_cost;
@@ -510,16 +588,19 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     if (!bu) {
       // without a count of this frontier in the tile tables (the probe
       // replaced them, or the count was fused away): a full compaction
-      launch_compact(g, w.qs, w.front[cur].as<uint32_t>(), c, s,
-                     probed || fused ? 0 : CMP_WRITE_ONLY);
+      LevelSide zs;
+      zs.zero = clear;
+      zs.nzero = clear ? words : 0;
+      launch_compact(g, w.qs, front, c, s, probed || fused ? 0 : CMP_WRITE_ONLY, nullptr, 0, 1,
+                     0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &zs);
       a.ctr = c;
       a.level_value = L + 2;
-      HIPC(hipEventRecord(w.ev[0], s));
+      if (timed) HIPC(hipEventRecord(w.ev[0], s));
       launch_expand(g, A_BFS, a, s);
-      HIPC(hipEventRecord(w.ev[1], s));
+      if (timed) HIPC(hipEventRecord(w.ev[1], s));
     } else {
-      if (!probed) probe(hint, next, L + 2);
-      HIPC(hipEventRecord(w.ev[0], s));
+      if (!probed) probe(front, hint, next, L + 2, clear);
+      if (timed) HIPC(hipEventRecord(w.ev[0], s));
       // the probe scanned the short lists already (and, with hints, settled
       // every degree-1 vertex by its hint) and wrote the long ones, per wave
       // region, for the early-exit kernel
@@ -530,17 +611,21 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       b.rbit = g.store.d_rbit.as<uint32_t>();
       b.eptr = g.store.d_eptr.as<uint64_t>();
       b.deg = g.d_deg.as<uint32_t>();
-      b.front = w.front[cur].as<uint32_t>();
+      b.front = front;
       b.vis = w.vis.as<uint32_t>();
       b.next = next;
       b.val = w.val.as<uint32_t>();
       b.rd_ctr = c_long;
       b.level_value = L + 2;
       b.nctr = c_next;
+      // published after it: slots 2-4 (records read, the short lists'
+      // counts, the next frontier's sums); c_next is reset
+      b.side.pub_src = c_long;
+      b.side.pub = w.h_ctr_dev + 2;
+      b.side.npub = 3;
+      b.side.zctr = c_next;
       launch_bottom_up(g, b, s);
-      HIPC(hipEventRecord(w.ev[1], s));
-      HIPC(hipMemcpyAsync(&w.h_ctr[2], c_long, 2 * sizeof(LevelCtr), hipMemcpyDeviceToHost, s));
-      if (sure && trace && hints) HIPC(hipMemcpyAsync(w.h_aux + 1, d_pc, 8, hipMemcpyDeviceToHost, s));
+      if (timed) HIPC(hipEventRecord(w.ev[1], s));
     }
     pend.valid = true;
     pend.bu = bu;
@@ -554,7 +639,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     pend.td_cost = td_cost;
     pend.bu_cost = bu_cost;
     fused_next = bu;
-    cur ^= 1;
+    cur = (cur + 1) % 3;
   }
   if (st) {
     st->ms = (now_ns() - t0) * 1e-6;
@@ -562,8 +647,7 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     st->expand_launches = launches;
     st->bottom_up_ms = bottom_up_ms;
     if (probes) {
-      HIPC(hipMemcpy(w.h_aux, d_pc, 6 * 8, hipMemcpyDeviceToHost));
-      const unsigned long long *pc = w.h_aux;
+      const unsigned long long *pc = w.h_aux + HAUX_PC;  // published by the last probe
       st->hint_hits = pc[0];
       st->probe_ms = probe_ms;
       st->probe_launches = probes;
@@ -586,19 +670,22 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   hipStream_t s = g.stream;
   const uint64_t words = g.n / 32 + 2;
   stats_begin(st);
-  launch_fill_u32(w.val.as<uint32_t>(), (uint32_t)(g.n + 1), g.n, s);
-  launch_fill_u32(w.val.as<uint32_t>() + src, 1u, 1, s);
   // on a symmetric graph a degree-0 vertex has no in-edges either: it
-  // starts visited (its level stays n+1), so bottom-up levels never look at it
-  if (g.sym && g.dir_opt)
-    HIPC(hipMemcpyAsync(w.vis.p, g.d_iso.p, words * 4, hipMemcpyDeviceToDevice, s));
-  else
-    HIPC(hipMemsetAsync(w.vis.p, 0, words * 4, s));
-  HIPC(hipMemsetAsync(w.front[0].p, 0, words * 4, s));
-  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.vis.as<uint32_t>(), src);
-  hipLaunchKernelGGL(k_set_bit, dim3(1), dim3(1), 0, s, w.front[0].as<uint32_t>(), src);
-  HIPC(hipStreamSynchronize(s));
-  if (g.sym && g.dir_opt) bfs_direction_optimizing(g, st);
+  // starts visited (its level stays n+1), so bottom-up levels never look at
+  // it.  The direction-optimising loop also starts from zero next-frontier
+  // sums (ctr slot 4) and probe counters (aux[4, 10)).
+  const bool dopt = g.sym && g.dir_opt;
+  hipLaunchKernelGGL(k_bfs_init, dim3(fill_grid(std::max<uint64_t>(g.n / 4, words))), dim3(256),
+                     0, s, w.val.as<uint32_t>(), g.n, src, w.vis.as<uint32_t>(),
+                     dopt ? g.d_iso.as<uint32_t>() : nullptr, w.front[0].as<uint32_t>(),
+                     w.front[1].as<uint32_t>(), words,
+                     dopt ? reinterpret_cast<unsigned long long *>(w.qs.ctr.as<LevelCtr>() + 4)
+                          : nullptr,
+                     dopt ? (uint32_t)(sizeof(LevelCtr) / 8) : 0u,
+                     dopt ? w.aux.as<unsigned long long>() + AUX_PC : nullptr,
+                     dopt ? 6u : 0u);
+  HIPC(hipGetLastError());
+  if (dopt) bfs_direction_optimizing(g, st);
   else value_loop(g, g.ws, A_BFS, st);
   finish_values(g, out, st, (uint32_t)(g.n + 1));
 }

@@ -19,6 +19,7 @@
 
 #include "internal.hpp"
 #include "scan.hpp"
+#include "side.hpp"
 
 namespace thv {
 
@@ -115,6 +116,13 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
   }
 }
 
+// The level's counters to the host-mapped readback once the early-exit
+// kernel has finished: one wave, one tiny launch in place of the copies.
+// (A last-block count inside k_bottom_up would need a device-scope
+// acquire-release per block, an L2 write-back and invalidate each, which
+// cost more than this launch.)
+__global__ void __launch_bounds__(64) k_publish(LevelSide s) { side_publish(s, threadIdx.x); }
+
 // Parent hints from the keys of the hint-build expansion: the low word of
 // a key is ~(neighbour of largest degree, smallest id among ties).
 __global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *key, uint64_t n,
@@ -129,9 +137,12 @@ __global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *k
 }  // namespace
 
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
+  if (a.side.zero) fail(THV_ESTATE, "the early-exit kernel clears no words");
   const int grid = g.shape.probe_bpc * g.cus;
   if (g.rec == 8) hipLaunchKernelGGL(k_bottom_up<8>, dim3(grid), dim3(BBS), 0, s, a);
   else hipLaunchKernelGGL(k_bottom_up<4>, dim3(grid), dim3(BBS), 0, s, a);
+  if (a.side.pub || a.side.xpub || a.side.zctr)
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, a.side);
   HIPC(hipGetLastError());
 }
 

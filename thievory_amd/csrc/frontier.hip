@@ -25,6 +25,7 @@
 
 #include "internal.hpp"
 #include "scan.hpp"
+#include "side.hpp"
 
 namespace thv {
 
@@ -63,6 +64,7 @@ struct CompactArgs {
   uint32_t fthr;
   uint32_t *clear;       // if set (write pass): clear the selected vertices' bits here
   uint32_t *minout;      // if set (count pass, with fval): atomicMin of the selected fval
+  LevelSide side;        // count pass: k_tcount clears, k_sscan publishes
 };
 
 // One wave's 1024-vertex sub-tile, read in two dependent memory round
@@ -143,6 +145,7 @@ __global__ void __launch_bounds__(CBS) k_tcount(CompactArgs a) {
   __shared__ unsigned long long s[CBS / 64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t fmin = 0xFFFFFFFFu;
+  side_clear(a.side);
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB;
     unsigned long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
@@ -206,7 +209,7 @@ constexpr int SNW = SBS / 64;
 // counters and clears the segment sums for the next count.  A tile's base
 // is its segment's base plus the tiles before it in the segment, which the
 // write pass adds itself (one coalesced load).
-__global__ void __launch_bounds__(SBS) k_sscan(CompactArgs a) {
+__device__ __forceinline__ void sscan_block(const CompactArgs &a) {
   __shared__ unsigned long long sh[4][SNW + 1];
   unsigned long long carry[4] = {0, 0, 0, 0};
   for (uint32_t b = 0; b < a.nseg; b += SBS) {
@@ -228,16 +231,30 @@ __global__ void __launch_bounds__(SBS) k_sscan(CompactArgs a) {
     }
     __syncthreads();
   }
+  __shared__ LevelCtr tot;
   if (threadIdx.x == 0) {
-    a.ctr->qn[0] = carry[0];
-    a.ctr->qn[1] = carry[1];
-    a.ctr->qw[0] = carry[2];
-    a.ctr->qw[1] = carry[3];
-    a.ctr->touched[0] = 0;
-    a.ctr->touched[1] = 0;
-    a.ctr->rd[0] = 0;
-    a.ctr->rd[1] = 0;
+    tot = *a.ctr;  // keeps cand
+    tot.qn[0] = carry[0];
+    tot.qn[1] = carry[1];
+    tot.qw[0] = carry[2];
+    tot.qw[1] = carry[3];
+    tot.touched[0] = 0;
+    tot.touched[1] = 0;
+    tot.rd[0] = 0;
+    tot.rd[1] = 0;
+    *a.ctr = tot;
   }
+  if (a.side.pub || a.side.xpub || a.side.zctr) {
+    __syncthreads();
+    if (threadIdx.x < 64) side_publish(a.side, threadIdx.x, a.ctr, &tot);
+  }
+}
+
+__global__ void __launch_bounds__(SBS) k_sscan(CompactArgs a) { sscan_block(a); }
+
+// Both tile-table sets of a probe in one launch: block k scans set k.
+__global__ void __launch_bounds__(SBS) k_sscan2(CompactArgs a0, CompactArgs a1) {
+  sscan_block(blockIdx.x ? a1 : a0);
 }
 
 // Base of a tile: its segment's base plus the tiles before it in the
@@ -392,6 +409,7 @@ struct ProbeArgs {
   // sweeping every vertex: the unvisited set only shrinks.
   const uint32_t *cin, *ccin;
   uint32_t *cout, *ccout;
+  LevelSide side;  // zero words only
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
@@ -451,6 +469,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0, ncout = 0;
   unsigned long long fe[2] = {0, 0};  // next frontier: edges (host, HBM)
   uint32_t fn[2] = {0, 0};            //   and entries
+  side_clear(a.side);
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB, w0 = base >> 5;
     const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
@@ -629,6 +648,7 @@ __global__ void __launch_bounds__(CBS) k_bu_probe_list(ProbeArgs a) {
   unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0, ncin = 0, ncout = 0;
   unsigned long long fe[2] = {0, 0};
   uint32_t fn[2] = {0, 0};
+  side_clear(a.side);
   for (uint32_t gi = blockIdx.x * (CBS / 64) + wv; gi < ngroups; gi += nwaves) {
     const uint32_t r0 = gi * LR;
     // lane j < LR: region r0 + j's candidates [oj, oj + cj) in the group
@@ -823,11 +843,13 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
 // tiles (SEG * 4 regions, one per thread), on top of the segment's base
 // from k_sscan (both residencies).  The last block writes the total after
 // the last region, so the early-exit kernel can split the lists evenly.
+// Block 0 also publishes the probe's counters (side), all final by now.
 struct LbaseArgs {
   const uint32_t *lcnt;
   const unsigned long long *sbase;  // set 1 (long lists) segment bases
   uint32_t *loff;
   uint32_t nregions, nseg;
+  LevelSide side;
 };
 constexpr int LBS = SEG * (CBS / 64);
 __global__ void __launch_bounds__(LBS) k_lbase(LbaseArgs a) {
@@ -839,6 +861,7 @@ __global__ void __launch_bounds__(LBS) k_lbase(LbaseArgs a) {
   const uint32_t base = (uint32_t)(a.sbase[blockIdx.x * 4] + a.sbase[blockIdx.x * 4 + 1]);
   if (rg < a.nregions) a.loff[rg] = base + ex;
   if (blockIdx.x == a.nseg - 1 && threadIdx.x == 0) a.loff[a.nregions] = base + tot;
+  if (blockIdx.x == 0 && threadIdx.x < 64) side_publish(a.side, threadIdx.x);
 }
 
 // Partitions stamped by this call, per residency.
@@ -860,7 +883,7 @@ __global__ void __launch_bounds__(CBS) k_touched(CompactArgs a) {
 void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode, uint32_t *val, uint32_t val_value, uint32_t dmin,
                     uint32_t dmax, int tset, const uint32_t *fval, uint32_t fthr,
-                    uint32_t *clear, uint32_t *minout) {
+                    uint32_t *clear, uint32_t *minout, const LevelSide *side) {
   if (dmin == 0) dmin = 1;  // degree-0 vertices never have work
   Store &st = g.store;
   CompactArgs a;
@@ -901,6 +924,7 @@ void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
   a.fthr = fthr;
   a.clear = clear;
   a.minout = minout;
+  if (side) a.side = *side;
   const uint32_t grid = a.ntiles < 65535u ? a.ntiles : 65535u;
   // WRITE_ONLY continues a COUNT_ONLY call on the same bitmap and counters:
   // the tile bases and totals are still in place
@@ -921,9 +945,14 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s, hipEvent_t t0,
                      hipEvent_t t1, LevelCtr *nctr, uint32_t *val, uint32_t level_value,
                      const uint32_t *cin, const uint32_t *ccin, uint32_t *cout,
-                     uint32_t *ccout) {
+                     uint32_t *ccout, const LevelSide *side) {
   ProbeArgs p;
   memset(&p, 0, sizeof p);
+  if (side && side->zctr) fail(THV_ESTATE, "the probe resets no counter");
+  if (side) {
+    p.side.zero = side->zero;
+    p.side.nzero = side->nzero;
+  }
   p.front = front;
   p.rbit = g.store.d_rbit.as<uint32_t>();
   p.deg = g.d_deg.as<uint32_t>();
@@ -985,8 +1014,9 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
   }
   if (t1) HIPC(hipEventRecord(t1, s));
   // tile bases and totals of both sets
+  CompactArgs sa2[2];
   for (int k = 0; k < 2; k++) {
-    CompactArgs a;
+    CompactArgs &a = sa2[k];
     memset(&a, 0, sizeof a);
     a.ntiles = p.ntiles;
     a.tcnt = w.tcnt[k].as<uint32_t>();
@@ -995,14 +1025,19 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
     a.sbase = w.sbase[k].as<unsigned long long>();
     a.nseg = (uint32_t)div_up(a.ntiles, SEG);
     a.ctr = k == 0 ? cs : cl;
-    hipLaunchKernelGGL(k_sscan, dim3(1), dim3(SBS), 0, s, a);
   }
+  hipLaunchKernelGGL(k_sscan2, dim3(2), dim3(SBS), 0, s, sa2[0], sa2[1]);
   LbaseArgs lb;
   lb.lcnt = p.lcnt;
   lb.sbase = w.sbase[1].as<unsigned long long>();
   lb.loff = w.loff.as<uint32_t>();
   lb.nregions = p.ntiles * (CBS / 64);
   lb.nseg = (uint32_t)div_up(p.ntiles, SEG);
+  if (side) {
+    lb.side = *side;
+    lb.side.zero = nullptr;
+    lb.side.nzero = 0;
+  }
   hipLaunchKernelGGL(k_lbase, dim3(lb.nseg), dim3(LBS), 0, s, lb);
   HIPC(hipGetLastError());
 }

@@ -452,10 +452,19 @@ void workspace_ensure(Graph &g) {
   w.vis.alloc(words * 4);
   w.aux.alloc(16 * sizeof(unsigned long long));
   w.val.alloc((g.n + 1) * 4);
-  if (!w.h_ctr)
-    HIPC(hipHostMalloc((void **)&w.h_ctr, CTR_SLOTS * sizeof(LevelCtr), hipHostMallocDefault));
-  if (!w.h_aux)
-    HIPC(hipHostMalloc((void **)&w.h_aux, 16 * sizeof(unsigned long long), hipHostMallocDefault));
+  // fine-grained: a kernel's stores land in host memory without a cache
+  // flush, and the host reads them after the stream sync
+  constexpr unsigned HF = hipHostMallocMapped | hipHostMallocCoherent;
+  if (!w.h_ctr) {
+    HIPC(hipHostMalloc((void **)&w.h_ctr, CTR_SLOTS * sizeof(LevelCtr), HF));
+    memset(w.h_ctr, 0, CTR_SLOTS * sizeof(LevelCtr));
+    HIPC(hipHostGetDevicePointer((void **)&w.h_ctr_dev, w.h_ctr, 0));
+  }
+  if (!w.h_aux) {
+    HIPC(hipHostMalloc((void **)&w.h_aux, 16 * sizeof(unsigned long long), HF));
+    memset(w.h_aux, 0, 16 * sizeof(unsigned long long));
+    HIPC(hipHostGetDevicePointer((void **)&w.h_aux_dev, w.h_aux, 0));
+  }
   for (auto &e : w.ev)
     if (!e) HIPC(hipEventCreate(&e));
   w.n = g.n;

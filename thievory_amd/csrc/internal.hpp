@@ -151,13 +151,22 @@ struct QueueSet {
 struct Workspace {
   uint64_t n = 0, m = 0;
   QueueSet qs;
-  DevMem front[2];            // frontier bitmaps (n bits, word padded)
+  // frontier bitmaps (n bits, word padded).  The direction-optimising BFS
+  // rotates all three: level L reads front[L % 3], writes front[(L+1) % 3]
+  // and one of its own kernels clears front[(L+2) % 3] for level L+1.  The
+  // other loops use [0] and [1].
+  DevMem front[3];
   DevMem vis;                 // BFS visited bitmap
   DevMem val;                 // u32 values (BFS / CC / SSSP)
   DevMem f0, f1, f2;          // PR arrays (fp32)
   DevMem aux;                 // misc device counters (16 x u64)
-  LevelCtr *h_ctr = nullptr;  // pinned readback
+  // pinned, mapped readback (CTR_SLOTS LevelCtr, 16 u64): the host reads
+  // them after a stream sync; kernels that finish a level write them
+  // directly through the device addresses h_ctr_dev / h_aux_dev
+  LevelCtr *h_ctr = nullptr;
   unsigned long long *h_aux = nullptr;
+  LevelCtr *h_ctr_dev = nullptr;
+  unsigned long long *h_aux_dev = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // [2], [3]: probe
   ~Workspace();
 };
@@ -207,6 +216,7 @@ struct Graph {
   uint32_t sssp_delta = 16;
   bool cc_union = true;       // CC of a symmetric graph by union-find
   uint32_t uf_max_steps = 1u << 20;  // union-find step guard (then label propagation)
+  bool kernel_timing = true;  // BFS: HIP events around its launches
   // PageRank push: the destinations that receive the most records (from a
   // 1-in-64 sample of the edge array), sorted, and their bitmap; blocks
   // accumulate adds to them in LDS.  Built on the first push call.
@@ -257,6 +267,27 @@ struct ExpandArgs {
   int rec;
 };
 
+// Level bookkeeping a launch does on the side, so a level needs no memset
+// or readback copy of its own (every field may be null / 0):
+//   zero[0, nzero): words the launch's grid clears (a bitmap no kernel of
+//     the launch reads or writes otherwise);
+//   pub_src -> pub: npub LevelCtr copied, once the launch's counters are
+//     final, to the host-mapped readback; xsrc -> xpub: nx u64 counters too;
+//   zctr: a LevelCtr reset to zero after that copy.
+// The host reads the readback after a stream sync: the launch's completion
+// orders its stores to host memory before the sync returns.
+struct LevelSide {
+  uint32_t *zero = nullptr;
+  uint64_t nzero = 0;
+  const LevelCtr *pub_src = nullptr;
+  LevelCtr *pub = nullptr;
+  uint32_t npub = 0;
+  const unsigned long long *xsrc = nullptr;
+  unsigned long long *xpub = nullptr;
+  uint32_t nx = 0;
+  LevelCtr *zctr = nullptr;
+};
+
 // Compaction modes: INVERT takes the vertices whose bit is CLEAR (the
 // unvisited set of a bottom-up BFS level), NO_CHUNKS skips the chunk-start
 // table, COUNT_ONLY stops after the totals.
@@ -267,13 +298,16 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 // fval / fthr: select only vertices with fval[v] < fthr.  clear: the write
 // pass clears the bits of the vertices it selects there (the front bitmap
 // itself, for a pending set).  minout (with fval): the count pass lowers
-// *minout to the smallest fval of a selected vertex.
+// *minout to the smallest fval of a selected vertex.  side (count pass
+// only; ignored by WRITE_ONLY): the count kernel clears side->zero, the
+// scan publishes after writing the totals and resets side->zctr.
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode = 0, uint32_t *val = nullptr,
                     uint32_t val_value = 0, uint32_t dmin = 1,
                     uint32_t dmax = 0xFFFFFFFFu, int tset = 0,
                     const uint32_t *fval = nullptr, uint32_t fthr = 0,
-                    uint32_t *clear = nullptr, uint32_t *minout = nullptr);
+                    uint32_t *clear = nullptr, uint32_t *minout = nullptr,
+                    const LevelSide *side = nullptr);
 // Bottom-up probe, one sweep over the vertices: every unvisited vertex
 // whose parent hint (hint may be null) is in front joins vis and next, and
 // so does every unvisited vertex of degree 1..D with a frontier vertex in
@@ -282,7 +316,9 @@ void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr
 // (degree > D) for a WRITE_ONLY compaction of the inverted vis bitmap.
 // cnt[0..3] accumulate hint hits, unvisited vertices, unvisited vertices
 // with a hint, and bitmap words written.  Events t0 / t1 (may be null)
-// bracket the probe kernel alone.
+// bracket the probe kernel alone.  side: the probe kernel clears
+// side->zero; the last launch (long-list bases, after the probe, the short
+// lists and both scans) publishes (no zctr here).
 void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vis,
                      uint32_t *next, const uint32_t *hint, uint32_t D, LevelCtr *cs,
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s,
@@ -290,7 +326,7 @@ void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vi
                      LevelCtr *nctr = nullptr, uint32_t *val = nullptr,
                      uint32_t level_value = 0, const uint32_t *cin = nullptr,
                      const uint32_t *ccin = nullptr, uint32_t *cout = nullptr,
-                     uint32_t *ccout = nullptr);
+                     uint32_t *ccout = nullptr, const LevelSide *side = nullptr);
 
 struct BottomUpArgs {
   const uint32_t *llist, *loff;  // the probe's long lists, per wave region,
@@ -303,6 +339,7 @@ struct BottomUpArgs {
   LevelCtr *rd_ctr;           // receives rd[2]
   uint32_t level_value;
   LevelCtr *nctr;             // next-frontier bookkeeping (see launch_bu_probe)
+  LevelSide side;             // published after the kernel (no zero words here)
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
 void build_parent_hints(Graph &g);
