@@ -330,7 +330,7 @@ static float event_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
+static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   Workspace &w = g.ws;
   hipStream_t s = g.stream;
   const uint64_t words = bitmap_words(g.n);
@@ -431,25 +431,34 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
   // Level bookkeeping, run after the next level's count has synchronised
   // the stream (one host round trip per level instead of two).
   struct LevelRec {
-    // sure: bottom-up without a readback of the probe's counts (see below)
-    bool valid = false, bu = false, probed = false, hinted = false, sure = false;
+    // sure: bottom-up without a readback of the probe's counts (see below);
+    // later: level 0 read no counts, h comes from the readback at its end
+    bool valid = false, bu = false, probed = false, hinted = false, sure = false, later = false;
     uint32_t L = 0;
     uint64_t nf = 0, mf = 0;
     LevelCtr h;
     double td_cost = 0, bu_cost = 0;
   } pend;
-  // the bottom-up cost of the last level if it was probed and ran
-  // bottom-up, else negative
-  double prev_bu_cost = -1;
   auto short_long_cost = [&](const LevelCtr &pl, const LevelCtr &ps) {
     return (double)ps.qw[0] * rec / link + (double)ps.qn[0] * SHORT_NS * 1e-9 +
            (double)pl.qn[0] * g.do_vertex_ns * 1e-9;
   };
+  // An upper bound of any level's bottom-up cost: every host list of 1..16
+  // records read in full, and every host list priced as a long one (a
+  // level's short host lists, full-scan degree <= 16, and long host lists
+  // are subsets of these)
+  const double bu_bound = (double)g.store.host_short_edges * rec / link +
+                          (double)g.store.host_short_lists * SHORT_NS * 1e-9 +
+                          (double)g.store.host_vertices * g.do_vertex_ns * 1e-9;
+  // a bound of this level's bottom-up cost: the last level's when it was
+  // probed and ran bottom-up (the unvisited set only shrinks), else bu_bound
+  double prev_bu_cost = bu_bound;
   auto finish_level = [&](LevelRec &p) {
     if (!p.valid) return;
+    if (p.later) p.h = w.h_ctr[p.L & 1];  // the next count pass republished it
     if (p.bu && (!p.probed || p.sure)) probe_done();
     if (p.sure) p.bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
-    prev_bu_cost = p.bu && p.probed ? p.bu_cost : -1;
+    prev_bu_cost = p.bu && p.probed ? std::min(p.bu_cost, bu_bound) : bu_bound;
     const float ms = timed ? event_ms(w.ev[0], w.ev[1]) : 0.0f;
     LevelCtr hs, hb;  // bottom-up: full-scan and early-exit sets
     memset(&hs, 0, sizeof hs);
@@ -514,6 +523,15 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
     // level's write pass touched), resets c_next and clears `dead` for the
     // next level; a fused level has its probe or compaction clear it.
     const bool fused = fused_next;
+    // Level 0 from a source whose whole list would not pay for a probe (or,
+    // all edges in HBM, would not pass the edge-ratio rule) runs top-down
+    // whatever its counts: count and write passes back to back, no readback.
+    const uint64_t d0 = L == 0 ? g.h_off[src + 1] - g.h_off[src] : 0;
+    const bool quick =
+        L == 0 && d0 > 0 &&
+        (host_share ? (double)d0 * rec / link + td_hbm_cost(1, d0) <=
+                          (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED
+                    : (double)d0 <= (double)(g.m - std::min<uint64_t>(g.m, d0)) / g.do_alpha);
     if (!fused) {
       LevelSide cs;
       cs.zero = dead;
@@ -522,8 +540,26 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
       cs.pub = w.h_ctr_dev;
       cs.npub = 2;
       cs.zctr = c_next;
-      launch_compact(g, w.qs, front, c, s, CMP_COUNT_ONLY, w.val.as<uint32_t>(), L + 1, 1,
-                     0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &cs);
+      launch_compact(g, w.qs, front, c, s, quick ? 0 : CMP_COUNT_ONLY, w.val.as<uint32_t>(),
+                     L + 1, 1, 0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &cs);
+    }
+    if (quick) {
+      a.next = next;
+      a.ctr = c;
+      a.level_value = L + 2;
+      if (timed) HIPC(hipEventRecord(w.ev[0], s));
+      launch_expand(g, A_BFS, a, s);
+      if (timed) HIPC(hipEventRecord(w.ev[1], s));
+      visited_edges += d0;
+      prev_nf = 1;
+      pend = LevelRec();
+      pend.valid = pend.later = true;
+      pend.L = L;
+      pend.nf = 1;
+      pend.mf = d0;
+      fused_next = false;
+      cur = (cur + 1) % 3;
+      continue;
     }
     HIPC(hipStreamSynchronize(s));
     finish_level(pend);
@@ -558,10 +594,11 @@ static void bfs_direction_optimizing(Graph &g, thv_stats *st) {
         hinted = hints;
         // the short and long unvisited host lists of this level are subsets
         // of the last level's (the unvisited set only shrinks, degrees are
-        // fixed), so its bottom-up cost is at most the last level's: if that
-        // is below this level's top-down cost, bottom-up wins without the
-        // readback (the counts are read with the level's bookkeeping)
-        if (prev_bu_cost >= 0 && prev_bu_cost < td_cost) {
+        // fixed), so its bottom-up cost is at most the last level's, and at
+        // most bu_bound: if that is below this level's top-down cost,
+        // bottom-up wins without the readback (the counts are read with the
+        // level's bookkeeping).  The choice is the one the counts would give.
+        if (prev_bu_cost < td_cost) {
           bu = sure = true;
         } else {
           HIPC(hipStreamSynchronize(s));  // the probe published slots 2-4
@@ -685,7 +722,7 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
                      dopt ? w.aux.as<unsigned long long>() + AUX_PC : nullptr,
                      dopt ? 6u : 0u);
   HIPC(hipGetLastError());
-  if (dopt) bfs_direction_optimizing(g, st);
+  if (dopt) bfs_direction_optimizing(g, src, st);
   else value_loop(g, g.ws, A_BFS, st);
   finish_values(g, out, st, (uint32_t)(g.n + 1));
 }

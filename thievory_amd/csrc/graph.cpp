@@ -335,7 +335,7 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   const uint64_t slab_base = (uint64_t)(uintptr_t)s.slab.p;
   const uint64_t side_base = (uint64_t)(uintptr_t)s.side.p;
   const uint64_t host_base = (uint64_t)(uintptr_t)g.h_edges.dev;
-  s.host_vertices = 0;
+  s.host_vertices = s.host_short_lists = s.host_short_edges = 0;
   uint64_t side_at = 0;
   for (uint32_t p = 0; p < s.P; p++) {
     uint64_t pe0 = g.h_off[s.pv[p]];
@@ -352,6 +352,10 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
       } else {
         eptr[v] = host_base + g.h_off[v] * rec;
         s.host_vertices += d != 0;
+        if (d >= 1 && d <= SIDE_DEGREE) {
+          s.host_short_lists++;
+          s.host_short_edges += d;
+        }
       }
     }
   }

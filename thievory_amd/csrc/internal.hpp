@@ -85,6 +85,8 @@ struct Store {
   uint32_t Pdev = 0;                 // partitions resident in HBM
   uint64_t dev_edges = 0;            // edges resident in HBM
   uint64_t host_vertices = 0;        // vertices with edges in host memory
+  uint64_t host_short_lists = 0;     //   of them with 1..16 records
+  uint64_t host_short_edges = 0;     //   and those lists' records
   std::vector<uint32_t> pv;          // P+1 first vertex of each partition
   std::vector<uint8_t> pres;         // P residency (1 = HBM)
   DevMem slab;                       // resident records
