@@ -481,11 +481,11 @@ def pr_block(T, O, args):
 
 # ------------------------------------------------------------ C5 serving ---
 
-def serve_stream(T, g, queries, rate, seed):
+def serve_stream(T, g, queries, rate, seed, sssp_engines=0):
     """Submits (algo, src) queries to a fresh server, at Poisson arrivals of
     `rate` per second (None: all at once); returns throughput, latency and
     the per-query results."""
-    srv = T.Server(g, max_lanes=64)
+    srv = T.Server(g, max_lanes=64, sssp_engines=sssp_engines)
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, len(queries)) if rate else np.zeros(len(queries))
     t0 = time.perf_counter()
@@ -626,8 +626,8 @@ def serving_block(T, O, args):
     log(f"bfs stream: {sat} / {pois}")
     # the mixed 50/50 BFS+SSSP stream of the configuration
     qm, cm = stream(nq, mix=True)
-    msat, rs = serve_stream(T, g, qm, None, 6)
-    mpois, rp = serve_stream(T, g, qm, 0.7 * msat["qps"], 7)
+    msat, rs = serve_stream(T, g, qm, None, 6, args.sssp_engines)
+    mpois, rp = serve_stream(T, g, qm, 0.7 * msat["qps"], 7, args.sssp_engines)
     if expect:
         msat["oracle"] = check_served(rs, qm, cm, expect)
         mpois["oracle"] = check_served(rp, qm, cm, expect)
@@ -692,6 +692,8 @@ def main():
                     help="keep the default launch shapes and partition size (no thv_tune)")
     ap.add_argument("--serve-scale", type=int, default=25)
     ap.add_argument("--serve-queries", type=int, default=20000)
+    ap.add_argument("--sssp-engines", type=int, default=0,
+                    help="SSSP engines of the mixed-stream server (0: the library's default)")
     ap.add_argument("--workload", choices=["all", "bfs", "sssp", "pr", "serve"], default="all",
                     help="all: the driver line (every block); the others print one block")
     args = ap.parse_args()

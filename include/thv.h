@@ -349,6 +349,8 @@ typedef struct {
   uint32_t max_lanes;       /* concurrent queries per algorithm, <= 64     */
   uint64_t scratch_bytes;   /* device memory admission budget (0 = auto)   */
   uint32_t keep_full;       /* 1: keep full value arrays for thv_serve_poll */
+  uint32_t sssp_engines;    /* 64-lane SSSP engines, each on its own stream
+                               (0 = 1), while the scratch budget holds them */
 } thv_serve_cfg;
 
 typedef struct {

@@ -188,14 +188,15 @@ def test_batch_rejects_bad_input(g14):
         g.batch("cc", np.array([0], np.uint32))
 
 
-def test_serving_alone_equals_mix(g14):
-    """Queries arrive while others run (8 lanes per engine, 300 queries):
-    each result equals the query run alone -- on the oracle and through the
-    plain single-source GPU path."""
+@pytest.mark.parametrize("sssp_engines", [1, 3])
+def test_serving_alone_equals_mix(g14, sssp_engines):
+    """Queries arrive while others run (8 lanes per engine, 300 queries, one
+    or three SSSP engines): each result equals the query run alone -- on the
+    oracle and through the plain single-source GPU path."""
     g, og = g14
-    _say("case: serving mix")
+    _say("case: serving mix, sssp engines", sssp_engines)
     g.store_config(int(0.5 * g.m * 8), 1 << 12)
-    srv = T.Server(g, max_lanes=8)
+    srv = T.Server(g, max_lanes=8, sssp_engines=sssp_engines)
     rng = np.random.default_rng(9)
     qs = [("bfs" if rng.random() < 0.5 else "sssp", int(rng.integers(0, og.n)))
           for _ in range(300)]

@@ -79,7 +79,7 @@ class Stats(C.Structure):
 
 class ServeCfg(C.Structure):
     _fields_ = [("max_lanes", C.c_uint32), ("scratch_bytes", C.c_uint64),
-                ("keep_full", C.c_uint32)]
+                ("keep_full", C.c_uint32), ("sssp_engines", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -422,9 +422,9 @@ class Graph64:
 class Server:
     """Continuous-batching query server over a resident graph."""
 
-    def __init__(self, graph, max_lanes=64, scratch_bytes=0, keep_full=False):
+    def __init__(self, graph, max_lanes=64, scratch_bytes=0, keep_full=False, sssp_engines=0):
         self.graph = graph
-        cfg = ServeCfg(max_lanes, scratch_bytes, 1 if keep_full else 0)
+        cfg = ServeCfg(max_lanes, scratch_bytes, 1 if keep_full else 0, sssp_engines)
         h = _vp()
         check(lib().thv_serve_start(graph._h, C.byref(cfg), C.byref(h)))
         self._h = h

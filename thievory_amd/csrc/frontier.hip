@@ -464,7 +464,7 @@ __device__ __forceinline__ void probe_flush(const ProbeArgs &a, unsigned long lo
 //   both split by residency.
 __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   __shared__ unsigned long long s[CBS / 64][10];
-  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned long long lt = lanemask_lt();
   unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0, ncout = 0;
   unsigned long long fe[2] = {0, 0};  // next frontier: edges (host, HBM)
@@ -472,15 +472,22 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   side_clear(a.side);
   for (uint32_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const uint64_t base = (uint64_t)tile * TILE + (uint64_t)wv * SUB, w0 = base >> 5;
-    const bool owner = lane < 32 && ((w0 + lane) << 5) < a.n;
-    const uint32_t vw = owner ? a.vis[w0 + lane] : 0xFFFFFFFFu;
-    const uint32_t rw = owner ? a.rbit[w0 + lane] : 0u;
+    // the sub-tile's arrays from wave-uniform bases, indexed by 32-bit
+    // offsets (scalar base + vector offset addressing); nin: its vertices
+    // below n
+    const uint32_t nin = base < a.n ? (uint32_t)min<uint64_t>(a.n - base, SUB) : 0u;
+    uint32_t *const visb = a.vis + w0;
+    uint32_t *const nextb = a.next + w0;
+    const uint32_t *const rbitb = a.rbit + w0;
+    const uint32_t *const degb = a.deg + base;
+    const bool owner = lane < 32 && lane * 32 < nin;
+    const uint32_t vw = owner ? visb[lane] : 0xFFFFFFFFu;
+    const uint32_t rw = owner ? rbitb[lane] : 0u;
     // fast path: no vertex of the block's tile is unvisited
     uint32_t uw = 0;
     if (owner) {
-      const uint64_t first = (w0 + lane) << 5;
       uw = ~vw;
-      if (a.n - first < 32) uw &= (1u << (a.n - first)) - 1u;
+      if (nin - lane * 32 < 32) uw &= (1u << (nin - lane * 32)) - 1u;
     }
     if (!__syncthreads_or(uw != 0)) {
       if (lane == 0) {
@@ -500,7 +507,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     for (uint32_t r = 0; r < ROUNDS; r++) {
       const int src = (int)(2 * r + (lane >> 5));
       const uint32_t bit = 1u << (lane & 31);
-      const bool in = base + r * 64 + lane < a.n;
+      const bool in = r * 64 + lane < nin;
       if (in && !((uint32_t)__shfl((int)vw, src, 64) & bit)) un |= 1u << r;
       if ((uint32_t)__shfl((int)rw, src, 64) & bit) dev |= 1u << r;
     }
@@ -508,10 +515,11 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
     uint32_t f = 0;  // bit r: round r's vertex is a hint hit
     if (a.hint) {
+      const uint32_t *const hintb = a.hint + base;
       uint32_t h[ROUNDS];
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++)
-        h[r] = ((un >> r) & 1u) ? a.hint[base + r * 64 + lane] : NO_HINT;
+        h[r] = ((un >> r) & 1u) ? hintb[r * 64 + lane] : NO_HINT;
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++) nhint += h[r] != NO_HINT;
       // the frontier words the hints name, loaded together (a vertex
@@ -533,13 +541,13 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     }
     uint32_t d[ROUNDS];
 #pragma unroll
-    for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? a.deg[base + r * 64 + lane] : 0u;
+    for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? degb[r * 64 + lane] : 0u;
     if (a.nctr && f) {
 #pragma unroll
       for (uint32_t r = 0; r < ROUNDS; r++) {
         if (!((f >> r) & 1u)) continue;
         const uint32_t q = (dev >> r) & 1u;
-        a.val[base + r * 64 + lane] = a.level_value;
+        (a.val + base)[r * 64 + lane] = a.level_value;
         fn[q] += 1;
         fe[q] += d[r];
         d[r] = 0;  // placed: no list to scan
@@ -559,14 +567,14 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
       const bool lg = d[r] >= dlo && d[r] > a.D;
       const unsigned long long b = __ballot(sh);  // every lane takes part
       const unsigned long long bl = __ballot(lg);
-      if (sh) a.slist[base + ns + __popcll(b & lt)] = (uint32_t)(base + r * 64 + lane);
-      if (lg) a.llist[base + nl + __popcll(bl & lt)] = (uint32_t)(base + r * 64 + lane);
+      if (sh) (a.slist + base)[ns + __popcll(b & lt)] = (uint32_t)base + r * 64 + lane;
+      if (lg) (a.llist + base)[nl + __popcll(bl & lt)] = (uint32_t)base + r * 64 + lane;
       ns += __popcll(b);
       nl += __popcll(bl);
       if (a.cout) {
         const bool c = (un >> r) & 1u;
         const unsigned long long bc = __ballot(c);
-        if (c) a.cout[base + nc + __popcll(bc & lt)] = (uint32_t)(base + r * 64 + lane);
+        if (c) (a.cout + base)[nc + __popcll(bc & lt)] = (uint32_t)base + r * 64 + lane;
         nc += __popcll(bc);
       }
       if (d[r] < dlo) continue;
@@ -584,8 +592,8 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     }
     hits += __popc(m);  // hint hits
     if (owner && m) {
-      a.vis[w0 + lane] = vw | m;
-      a.next[w0 + lane] |= m;
+      visb[lane] = vw | m;
+      nextb[lane] |= m;
       nwords++;
     }
     // s[w][set * 4 + i]: i = 0, 1 entries (host, HBM), 2, 3 edges

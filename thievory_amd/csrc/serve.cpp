@@ -224,7 +224,14 @@ int thv_serve_start(thv_graph *gh, const thv_serve_cfg *cfg, thv_server **out) {
     if (g->rec == 8) {
       uint32_t sl = lanes;
       while (sl && MultiEngine::bytes_needed(*g, THV_ALGO_SSSP, sl, kf) > budget - need_bfs) sl /= 2;
-      if (sl) s->eng.emplace_back(new MultiEngine(*g, THV_ALGO_SSSP, sl, kf));
+      uint64_t used = need_bfs;
+      const uint32_t want = std::max<uint32_t>(1, cfg->sssp_engines);
+      for (uint32_t k = 0; sl && k < want; k++) {
+        const uint64_t one_s = MultiEngine::bytes_needed(*g, THV_ALGO_SSSP, sl, kf);
+        if (k && used + one_s > budget) break;
+        s->eng.emplace_back(new MultiEngine(*g, THV_ALGO_SSSP, sl, kf));
+        used += one_s;
+      }
     }
     g->servers++;
     s->t0 = now_ns();
