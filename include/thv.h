@@ -173,6 +173,12 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       their time (thv_stats expand_ms, probe_ms,
  *                       bottom_up_ms); 0: no events, those stay 0 (each
  *                       event is a few microseconds of queue time).
+ *   THV_OPT_STORE_LAYOUT which partitions a split store keeps in HBM: 0
+ *                       (default) densest first, or sparsest first with the
+ *                       short host lists copied to HBM when the graph holds
+ *                       parent hints; 1 densest first; 2 sparsest first with
+ *                       the short-list side area.  Changing it rebuilds the
+ *                       store with the same budget.
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -187,6 +193,7 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_CC_UNION 11
 #define THV_OPT_CC_UF_STEPS 12
 #define THV_OPT_KERNEL_TIMING 13
+#define THV_OPT_STORE_LAYOUT 14
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 
@@ -310,8 +317,10 @@ int thv_autotune(thv_graph *g, const char *cache_path, int *chosen);
  *   multi-source grid (4/8/16 blocks per CU; batches),
  *   bottom-up full-scan degree (8/16; BFS on symmetric graphs),
  *   SSSP bucket width (8/16/32/64),
- *   partition size (2^14, 2^16, 2^18 edges; with partitions != 0, each
- *   rebuilds the store under the same HBM budget).
+ *   with partitions != 0, each rebuilding the store under the same HBM
+ *   budget: the store layout (the other of densest / sparsest partitions
+ *   first, THV_OPT_STORE_LAYOUT) and the partition size (2^14, 2^16, 2^18
+ *   edges).
  * Every candidate's results are compared with the starting configuration's
  * (bit-exact digests; PageRank within 1e-5 normalised L1): a difference is
  * THV_ESTATE.  The choice stays on the handle and, keyed by device, graph
@@ -340,6 +349,7 @@ typedef struct {
   double default_ms;         /* median workload time, starting settings     */
   double tuned_ms;           /*   and chosen settings                        */
   int from_cache;
+  int store_layout;          /* THV_OPT_STORE_LAYOUT chosen (with partitions) */
 } thv_tune_result;
 int thv_tune(thv_graph *g, const thv_tune_cfg *cfg, thv_tune_result *out);
 

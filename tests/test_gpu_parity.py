@@ -191,6 +191,36 @@ def test_bfs_kernel_timing_off(rmat16, frac):
         g.set_option("kernel_timing", 1)
 
 
+@pytest.mark.parametrize("layout", [1, 2])
+def test_store_layouts_exact(layout):
+    """Both residency orders of a split store -- densest partitions first,
+    or sparsest first with the short host lists copied to HBM -- give the
+    oracle's SSSP distances, BFS levels and CC labels on 8-byte records, and
+    BFS levels on the symmetric 4-byte graph with parent hints."""
+    _say("case: store layout", layout)
+    og = O.rmat(15, 16, seed=3, weights=True)
+    g = T.Graph.rmat(15, 16, seed=3, weights=True)
+    g.store_config(int(0.3 * g.m * 8), 1 << 12)
+    auto = g.store_info()
+    g.set_option("store_layout", layout)
+    assert g.get_option("store_layout") == layout
+    info = g.store_info()
+    assert 0 < info["device_edges"] < g.m
+    # 8-byte records choose densest first by themselves
+    assert (info == auto) == (layout == 1)
+    for s in (1, 77, 4321):
+        assert np.array_equal(g.sssp(s)[0], O.sssp(og, s)), s
+        assert np.array_equal(g.bfs(s)[0], O.bfs(og, s)), s
+    assert np.array_equal(g.cc()[0], O.cc(og))
+    og4 = O.rmat(15, 16, seed=3)
+    g4 = T.Graph.rmat(15, 16, seed=3)
+    g4.store_config(int(0.25 * g4.m * 4), 1 << 12)
+    g4.set_option("store_layout", layout)
+    assert g4.get_option("hint_bytes") > 0
+    for s in (1, 77, 4321):
+        assert np.array_equal(g4.bfs(s)[0], O.bfs(og4, s)), s
+
+
 def _beads(k, s):
     """k cliques of s vertices in a row, clique b's last vertex joined to
     clique b+1's first: from vertex 0 the frontier alternates between one

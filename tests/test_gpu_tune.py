@@ -37,8 +37,10 @@ def test_tune_bfs_split_store_with_partitions(tmp_path):
     r = g.tune("bfs", partitions=True, cache_path=cache)
     _valid(r, "bfs")
     # every knob's candidates were timed: 5 shapes, 3 probe grids, 2 degrees,
-    # 3 partition sizes, less the starting value of each, plus the start
-    assert r["candidates"] == 1 + 4 + 2 + 1 + 2 and r["from_cache"] == 0
+    # 2 store layouts, 3 partition sizes, less the starting value of each,
+    # plus the start
+    assert r["candidates"] == 1 + 4 + 2 + 1 + 1 + 2 and r["from_cache"] == 0
+    assert r["store_layout"] in (0, 1)
     assert g.store_info()["partitions"] > 0
     for s in (1, 5, 4097):
         assert np.array_equal(g.bfs(s)[0], O.bfs(og, s)), s
@@ -56,7 +58,9 @@ def test_tune_sssp_and_batches():
     g.store_config(int(0.3 * g.m * 8), 1 << 14)
     r = g.tune("sssp", sources=[1, 2, 3], partitions=True, repeats=2)
     _valid(r, "sssp")
-    assert r["candidates"] == 1 + 3 + 4 + 2
+    assert r["candidates"] == 1 + 3 + 4 + 1 + 2
+    assert r["store_layout"] in (0, 2)
+    assert g.get_option("store_layout") == r["store_layout"]
     for s in (1, 9):
         assert np.array_equal(g.sssp(s)[0], O.sssp(og, s)), s
     srcs = np.arange(64, dtype=np.uint32) * 7

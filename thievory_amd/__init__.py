@@ -29,7 +29,7 @@ PR_MODE = {"push": 0, "pull": 1}
 OPTIONS = {"symmetric": 1, "direction": 2, "do_alpha": 3, "do_beta": 4, "do_vertex_ns": 5,
            "do_link_gbps": 6, "do_pull_degree": 7, "parent_hints": 8, "hint_bytes": 9,
            "sssp_delta": 10, "cc_union": 11,
-           "cc_uf_steps": 12, "kernel_timing": 13}
+           "cc_uf_steps": 12, "kernel_timing": 13, "store_layout": 14}
 ERRORS = {0: "OK", -1: "EINVAL", -2: "EIO", -3: "ENOMEM", -4: "EHIP", -5: "EFORMAT",
           -6: "EBUSY", -7: "ENOTSUP", -8: "ESTATE"}
 
@@ -56,7 +56,8 @@ class TuneResult(C.Structure):
                 ("probe_blocks_per_cu", C.c_int), ("batch_blocks_per_cu", C.c_int),
                 ("partition_edges", C.c_uint32), ("sssp_delta", C.c_uint32),
                 ("pull_degree", C.c_uint32), ("candidates", C.c_uint32),
-                ("default_ms", C.c_double), ("tuned_ms", C.c_double), ("from_cache", C.c_int)]
+                ("default_ms", C.c_double), ("tuned_ms", C.c_double), ("from_cache", C.c_int),
+                ("store_layout", C.c_int)]
 
 
 class Stats(C.Structure):

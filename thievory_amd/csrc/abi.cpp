@@ -79,8 +79,7 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
     std::lock_guard<std::mutex> lk(g->mu);
     // options that decide the parent hints may rebuild the store
     auto refresh = [&] {
-      if (g->servers.load() > 0 && g->store.configured &&
-          store_wants_hints(*g, g->store.budget) != g->store.hint_reserved)
+      if (g->servers.load() > 0 && store_stale(*g))
         fail(THV_EBUSY, "stop the servers first: this change rebuilds the store");
     };
     switch (option) {
@@ -136,6 +135,14 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
         g->do_pull_degree = (uint32_t)value;
         break;
       case THV_OPT_KERNEL_TIMING: g->kernel_timing = value != 0.0; break;
+      case THV_OPT_STORE_LAYOUT: {
+        NEED(value == 0.0 || value == 1.0 || value == 2.0);
+        const int old = g->store_layout;
+        g->store_layout = (int)value;
+        try { refresh(); } catch (...) { g->store_layout = old; throw; }
+        store_refresh(*g);
+        break;
+      }
       default: fail(THV_EINVAL, "unknown option");
     }
   });
@@ -159,6 +166,7 @@ int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
       case THV_OPT_CC_UNION: *value = g->cc_union ? 1.0 : 0.0; break;
       case THV_OPT_CC_UF_STEPS: *value = g->uf_max_steps; break;
       case THV_OPT_KERNEL_TIMING: *value = g->kernel_timing ? 1.0 : 0.0; break;
+      case THV_OPT_STORE_LAYOUT: *value = g->store_layout; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });

@@ -31,16 +31,19 @@ namespace {
 struct Knobs {
   int bpc, ept, pbpc, bbpc;
   uint32_t part, delta, pull;
+  int layout;  // THV_OPT_STORE_LAYOUT
 };
 
 Knobs current(const Graph &g) {
   return {g.shape.blocks_per_cu, g.shape.ept, g.shape.probe_bpc, g.shape.batch_bpc,
-          g.store.part_edges,    g.sssp_delta, g.do_pull_degree};
+          g.store.part_edges,    g.sssp_delta, g.do_pull_degree, g.store_layout};
 }
 
 void apply(Graph &g, const Knobs &k) {
   store_ensure(g);
-  if (k.part != g.store.part_edges) store_configure(g, g.store.budget, k.part);
+  g.store_layout = k.layout;
+  if (k.part != g.store.part_edges || store_stale(g))
+    store_configure(g, g.store.budget, k.part);
   g.shape.blocks_per_cu = k.bpc;
   g.shape.ept = k.ept;
   g.shape.probe_bpc = k.pbpc;
@@ -168,7 +171,7 @@ std::string knob_text(const Knobs &k, double d_ms, double t_ms) {
   std::ostringstream o;
   o.precision(17);  // the times read back exactly
   o << k.bpc << " " << k.ept << " " << k.pbpc << " " << k.bbpc << " " << k.part << " " << k.delta
-    << " " << k.pull << " " << d_ms << " " << t_ms;
+    << " " << k.pull << " " << k.layout << " " << d_ms << " " << t_ms;
   return o.str();
 }
 
@@ -183,9 +186,10 @@ bool cache_read(const char *path, const std::string &key, Knobs &k, double &d_ms
     std::istringstream in(line.substr(bar + 1));
     Knobs x;
     double a = 0, b = 0;
-    if (in >> x.bpc >> x.ept >> x.pbpc >> x.bbpc >> x.part >> x.delta >> x.pull >> a >> b &&
+    if (in >> x.bpc >> x.ept >> x.pbpc >> x.bbpc >> x.part >> x.delta >> x.pull >> x.layout >> a >>
+            b &&
         x.bpc > 0 && (x.ept == 4 || x.ept == 8) && x.pbpc > 0 && x.bbpc > 0 && x.part > 0 &&
-        x.pull <= 16) {
+        x.pull <= 16 && x.layout >= 0 && x.layout <= 2) {
       k = x;
       d_ms = a;
       t_ms = b;
@@ -313,6 +317,25 @@ void tune_impl(Graph &g, const thv_tune_cfg &cfg, thv_tune_result *res, bool sha
             v.push_back(k);
           }
       });
+    // the store layouts (densest or sparsest partitions first) that differ
+    // from the current one in effect
+    if (!shape_only && cfg.partitions)
+      vary([&](std::vector<Knobs> &v) {
+        const bool hints = store_wants_hints(g, g.store.budget);
+        const int was = g.store_layout;
+        g.store_layout = best.layout;
+        const bool cur = store_sparse_first(g, hints);
+        for (int l : {0, 1, 2}) {
+          g.store_layout = l;
+          if (l != best.layout && store_sparse_first(g, hints) != cur) {
+            Knobs k = best;
+            k.layout = l;
+            v.push_back(k);
+            break;  // the other order: one candidate
+          }
+        }
+        g.store_layout = was;
+      });
     if (!shape_only && cfg.partitions)
       vary([&](std::vector<Knobs> &v) {
         for (uint32_t p : {1u << 14, 1u << 16, 1u << 18})
@@ -341,6 +364,7 @@ void tune_impl(Graph &g, const thv_tune_cfg &cfg, thv_tune_result *res, bool sha
     res->default_ms = d_ms;
     res->tuned_ms = t_ms;
     res->from_cache = cached ? 1 : 0;
+    res->store_layout = best.layout;
   }
 }
 

@@ -168,11 +168,22 @@ bool store_wants_hints(const Graph &g, uint64_t budget) {
   return g.rec == 4 && budget >= 2 * 4 * g.n;
 }
 
-// Options that decide the hints changed: rebuild with the same budget.
+bool store_sparse_first(const Graph &g, bool hints) {
+  return g.store_layout == 2 || (g.store_layout == 0 && hints);
+}
+
+bool store_stale(const Graph &g) {
+  const Store &s = g.store;
+  if (!s.configured) return false;
+  const bool hints = store_wants_hints(g, s.budget);
+  return hints != s.hint_reserved || store_sparse_first(g, hints) != s.sparse_first;
+}
+
+// Options that decide the hints or the layout changed: rebuild with the
+// same budget.
 void store_refresh(Graph &g) {
   Store &s = g.store;
-  if (s.configured && store_wants_hints(g, s.budget) != s.hint_reserved)
-    store_configure(g, s.budget, s.part_edges);
+  if (store_stale(g)) store_configure(g, s.budget, s.part_edges);
 }
 
 void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
@@ -200,7 +211,8 @@ void store_configure(Graph &g, uint64_t budget, uint32_t part_edges) {
   // only from small frontiers or never (hints), stay on the host.
   // (Sparsest first counts only vertices with edges: isolated vertices
   // cost no request.)
-  const bool sparse_first = s.hint_reserved;
+  const bool sparse_first = store_sparse_first(g, s.hint_reserved);
+  s.sparse_first = sparse_first;
   std::vector<uint32_t> order(s.P), pverts(s.P);
   std::iota(order.begin(), order.end(), 0u);
   auto pedges = [&](uint32_t p) { return g.h_off[s.pv[p + 1]] - g.h_off[s.pv[p]]; };

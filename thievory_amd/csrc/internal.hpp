@@ -100,6 +100,7 @@ struct Store {
   // bytes out of the budget; a resident store holds them beside the lists.
   bool hint_reserved = false;
   DevMem d_hint;
+  bool sparse_first = false;         // residency ranked sparsest partitions first
 };
 
 constexpr uint32_t NO_HINT = 0xFFFFFFFFu;
@@ -219,6 +220,9 @@ struct Graph {
   bool cc_union = true;       // CC of a symmetric graph by union-find
   uint32_t uf_max_steps = 1u << 20;  // union-find step guard (then label propagation)
   bool kernel_timing = true;  // BFS: HIP events around its launches
+  // store residency order: 0 chosen by the hints (sparsest first with them),
+  // 1 densest partitions first, 2 sparsest first with the short-list side area
+  int store_layout = 0;
   // PageRank push: the destinations that receive the most records (from a
   // 1-in-64 sample of the edge array), sorted, and their bitmap; blocks
   // accumulate adds to them in LDS.  Built on the first push call.
@@ -234,6 +238,10 @@ void graph_build_tables(Graph &g);
 void store_configure(Graph &g, uint64_t budget, uint32_t part_edges);
 void store_ensure(Graph &g);
 bool store_wants_hints(const Graph &g, uint64_t budget);
+// whether the store would rank partitions sparsest first (with `hints`)
+bool store_sparse_first(const Graph &g, bool hints);
+// whether the current options would build a different store
+bool store_stale(const Graph &g);
 void store_refresh(Graph &g);
 void workspace_ensure(Graph &g);
 uint64_t now_ns();
