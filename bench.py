@@ -62,6 +62,9 @@ def mixed_roofline_ms(hbm_bytes, host_bytes):
     return max(hbm_bytes / HBM_PEAK, host_bytes / LINK_PEAK) * 1e3
 
 
+LAYOUTS = {0: "auto", 1: "densest-first", 2: "sparsest-first+side"}
+
+
 def tuned(r, seconds):
     """The tuner's choice (thv_tune) as reported: the settings, the median
     workload time with the starting and the chosen settings, and the
@@ -72,8 +75,7 @@ def tuned(r, seconds):
             "batch_blocks_per_cu": r["batch_blocks_per_cu"],
             "partition_edges": r["partition_edges"], "sssp_delta": r["sssp_delta"],
             "pull_degree": r["pull_degree"],
-            "store_layout": {0: "auto", 1: "densest-first", 2: "sparsest-first+side"}[
-                r["store_layout"]],
+            "store_layout": LAYOUTS[r["store_layout"]],
             "default_ms": round(r["default_ms"], 4),
             "tuned_ms": round(r["tuned_ms"], 4),
             "speedup": round(r["default_ms"] / r["tuned_ms"], 4) if r["tuned_ms"] else None,
@@ -404,6 +406,8 @@ def sssp_block(T, O, args):
     ef 16, 8-byte {dst, weight} records, weights in [1, 64], 30% HBM budget."""
     g = T.Graph.rmat(args.scale, 16, seed=1, symmetric=True, weights=True)
     g.store_config(int(0.30 * g.m * 8), args.partition_edges)
+    if args.sssp_layout:   # a fixed layout (the tuner may still change it)
+        g.set_option("store_layout", {"dense": 1, "sparse": 2}[args.sssp_layout])
     deg = np.diff(g.offsets())
     rng = np.random.default_rng(1)
     src = int(rng.choice(np.nonzero(deg)[0]))
@@ -419,7 +423,8 @@ def sssp_block(T, O, args):
     for _ in range(args.sssp_steps):
         stats.append(g.sssp(src, out=False)[1])
     out = {"workload": f"sssp-rmat{args.scale}-ef16-w8-budget30pct", "source": src,
-           "vertices": g.n, "edges": g.m, "store": g.store_info(), "tuning": tuning}
+           "vertices": g.n, "edges": g.m, "store": g.store_info(),
+           "store_layout": LAYOUTS[int(g.get_option("store_layout"))], "tuning": tuning}
     out.update(step_summary(stats, time.perf_counter() - t0, "k_expand<SSSP>"))
     dist, _ = g.sssp(src)
     # algorithm-independent rate: edges out of the reached vertices per
@@ -697,6 +702,8 @@ def main():
                     help="keep the default launch shapes and partition size (no thv_tune)")
     ap.add_argument("--serve-scale", type=int, default=25)
     ap.add_argument("--serve-queries", type=int, default=20000)
+    ap.add_argument("--sssp-layout", choices=["dense", "sparse"], default=None,
+                    help="C3 store layout before tuning (default: the library's)")
     ap.add_argument("--sssp-engines", type=int, default=0,
                     help="SSSP engines of the mixed-stream server (0: the library's default)")
     ap.add_argument("--workload", choices=["all", "bfs", "sssp", "pr", "serve"], default="all",
