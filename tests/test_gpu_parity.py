@@ -422,16 +422,19 @@ def test_sssp_delta_stepping(rmat16, frac):
 
 
 @pytest.mark.parametrize("scale,ef,frac", [(16, 16, 1.0), (16, 16, 0.25), (14, 1, 0.5),
-                                           (12, 2, 0.0)])
+                                           (12, 2, 0.0), (14, 4, 0.6), (15, 8, 0.3)])
 def test_cc_union_find(scale, ef, frac):
     """Union-find CC of a symmetric graph gives the component minima, the
     labels of min-label propagation: equal to the oracle and to the
     propagation path, on a giant-component R-MAT and on sparse ones with
-    many components."""
+    many components; with parent hints (split 4-byte stores of at least 8n
+    bytes) a host list's vertex links its hint first."""
     _say("case: cc union-find", scale, ef, frac)
     g = T.Graph.rmat(scale, ef, seed=5, symmetric=True)
     og = O.rmat(scale, ef, seed=5)
     g.store_config(int(frac * g.m * g.edge_bytes), 1024)
+    if (scale, ef) in ((16, 16), (14, 4), (15, 8)) and frac < 1.0:
+        assert g.get_option("hint_bytes") > 0
     ref = O.cc(og)
     assert g.get_option("cc_union") == 1.0
     lab, st = g.cc()

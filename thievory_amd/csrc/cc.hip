@@ -6,11 +6,14 @@
 // graph its fixpoint labels every vertex with the smallest id of its
 // component.  Union-find with "larger root under smaller" hooking (uf.hpp)
 // ends with exactly those labels, and reads far fewer edges:
-//   1. link every vertex to its first two neighbours, compress;
+//   1. link every vertex to its first two neighbours, compress (a vertex
+//      whose list is on the host links its parent hint instead, when the
+//      store holds hints: one HBM word, no request over the link);
 //   2. sample 1024 vertices: the most frequent root is the giant component;
 //   3. link the remaining lists of the vertices outside it (its own
 //      members need nothing more: an edge to a non-member is linked from
-//      the other side, which the graph's symmetry guarantees), compress.
+//      the other side, which the graph's symmetry guarantees; a hinted
+//      host list counts as remaining whatever its length), compress.
 // Directed graphs keep label propagation (the labels there are not
 // components).
 #include <algorithm>
@@ -27,17 +30,23 @@ namespace {
 constexpr int UBS = 256;
 constexpr uint32_t CC_SAMPLES = 1024;
 
-// Step 1, one vertex per thread: its first two records, then the links.
+// Step 1, one vertex per thread: its first two records, then the links;
+// with hints (nullable), a host list's vertex links its hint instead.
 template <int REC>
 __global__ void __launch_bounds__(UBS)
-k_cc_first(const uint64_t *eptr, const uint32_t *deg, const uint32_t *rbit, uint64_t n,
-           uint32_t *comp, uint32_t *err, uint32_t steps, unsigned long long *cnt) {
+k_cc_first(const uint64_t *eptr, const uint32_t *deg, const uint32_t *rbit, const uint32_t *hint,
+           uint64_t n, uint32_t *comp, uint32_t *err, uint32_t steps, unsigned long long *cnt) {
   __shared__ unsigned long long sh[UBS / 64];
-  unsigned long long rd[2] = {0, 0};  // records read: host, HBM
+  unsigned long long rd[2] = {0, 0};  // records read (host, HBM); a hint counts as HBM
   for (uint64_t v = blockIdx.x * (uint64_t)UBS + threadIdx.x; v < n;
        v += (uint64_t)gridDim.x * UBS) {
     const uint32_t d = deg[v];
     if (!d) continue;
+    if (hint && !((rbit[v >> 5] >> (v & 31)) & 1u)) {
+      uf_link((uint32_t)v, hint[v], comp, err, steps);  // d > 0: a neighbour
+      rd[1] += 1;
+      continue;
+    }
     const unsigned long long b = eptr[v];
     const uint32_t w0 = *reinterpret_cast<const uint32_t *>(b);
     const uint32_t w1 = d > 1 ? *reinterpret_cast<const uint32_t *>(b + REC) : w0;
@@ -65,15 +74,21 @@ __global__ void k_cc_sample(const uint32_t *comp, uint64_t n, uint32_t *out) {
   if (i < CC_SAMPLES) out[i] = comp[mix64(i) % n];
 }
 
-// Vertices whose remaining lists are linked in step 3: more than two
-// records and a root other than the giant's.  One lane per vertex; lanes 0
-// and 32 write the wave's two bitmap words.
+// Vertices whose remaining lists are linked in step 3: a root other than
+// the giant's, and more than two records or (hinted) a host list.  One lane
+// per vertex; lanes 0 and 32 write the wave's two bitmap words.
 __global__ void __launch_bounds__(UBS)
-k_cc_rest(const uint32_t *comp, const uint32_t *deg, uint64_t n, uint32_t giant, uint32_t *bits) {
+k_cc_rest(const uint32_t *comp, const uint32_t *deg, const uint32_t *rbit, bool hinted,
+          uint64_t n, uint32_t giant, uint32_t *bits) {
   const uint32_t lane = threadIdx.x & 63;
   for (uint64_t v = blockIdx.x * (uint64_t)UBS + threadIdx.x; v - lane < n;
        v += (uint64_t)gridDim.x * UBS) {
-    const bool sel = v < n && deg[v] > 2 && comp[v] != giant;
+    bool sel = false;
+    if (v < n) {
+      const uint32_t d = deg[v];
+      const bool rest = d > 2 || (hinted && d > 0 && !((rbit[v >> 5] >> (v & 31)) & 1u));
+      sel = rest && comp[v] != giant;
+    }
     const unsigned long long b = __ballot(sel);
     if (lane == 0 || lane == 32) {
       const uint64_t w = v >> 5;
@@ -101,12 +116,13 @@ bool cc_union_find(Graph &g, thv_stats *st) {
   const uint32_t steps = g.uf_max_steps;
   const uint64_t *eptr = g.store.d_eptr.as<uint64_t>();
   const uint32_t *deg = g.d_deg.as<uint32_t>(), *rbit = g.store.d_rbit.as<uint32_t>();
+  const uint32_t *hint = g.store.d_hint.as<uint32_t>();  // null without hints
   if (g.rec == 8)
-    hipLaunchKernelGGL(k_cc_first<8>, dim3(grid), dim3(UBS), 0, s, eptr, deg, rbit, n, comp, err,
-                       steps, aux);
+    hipLaunchKernelGGL(k_cc_first<8>, dim3(grid), dim3(UBS), 0, s, eptr, deg, rbit, hint, n, comp,
+                       err, steps, aux);
   else
-    hipLaunchKernelGGL(k_cc_first<4>, dim3(grid), dim3(UBS), 0, s, eptr, deg, rbit, n, comp, err,
-                       steps, aux);
+    hipLaunchKernelGGL(k_cc_first<4>, dim3(grid), dim3(UBS), 0, s, eptr, deg, rbit, hint, n, comp,
+                       err, steps, aux);
   hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(UBS), 0, s, comp, n, err, steps);
   // the sample lands in the probe's short-list buffer (unused here; at
   // least one tile = 16 KB, CC_SAMPLES x 4 = 4 KB), then on the host
@@ -127,7 +143,8 @@ bool cc_union_find(Graph &g, thv_stats *st) {
   // step 3: the rest of the lists outside the giant component
   uint32_t *bits = w.front[0].as<uint32_t>();
   HIPC(hipMemsetAsync(bits, 0, words * 4, s));
-  hipLaunchKernelGGL(k_cc_rest, dim3(grid), dim3(UBS), 0, s, comp, deg, n, giant, bits);
+  hipLaunchKernelGGL(k_cc_rest, dim3(grid), dim3(UBS), 0, s, comp, deg, rbit, hint != nullptr, n,
+                     giant, bits);
   LevelCtr *c = w.qs.ctr.as<LevelCtr>();
   launch_compact(g, w.qs, bits, c, s);
   ExpandArgs a = expand_args(g, w.qs);
