@@ -271,6 +271,25 @@ void tune_impl(Graph &g, const thv_tune_cfg &cfg, thv_tune_result *res, bool sha
       set(v);
       try_knob(v);
     };
+    // first the store layout (the other of densest / sparsest partitions
+    // first), which changes what every later knob is timed on
+    if (!shape_only && cfg.partitions)
+      vary([&](std::vector<Knobs> &v) {
+        const bool hints = store_wants_hints(g, g.store.budget);
+        const int was = g.store_layout;
+        g.store_layout = best.layout;
+        const bool cur = store_sparse_first(g, hints);
+        for (int l : {0, 1, 2}) {
+          g.store_layout = l;
+          if (l != best.layout && store_sparse_first(g, hints) != cur) {
+            Knobs k = best;
+            k.layout = l;
+            v.push_back(k);
+            break;  // the other order: one candidate
+          }
+        }
+        g.store_layout = was;
+      });
     if (!shape_only && (kind == THV_TUNE_SSSP || kind == THV_TUNE_BATCH_SSSP))
       vary([&](std::vector<Knobs> &v) {
         for (uint32_t d : {8u, 16u, 32u, 64u})
@@ -316,25 +335,6 @@ void tune_impl(Graph &g, const thv_tune_cfg &cfg, thv_tune_result *res, bool sha
             k.bbpc = p;
             v.push_back(k);
           }
-      });
-    // the store layouts (densest or sparsest partitions first) that differ
-    // from the current one in effect
-    if (!shape_only && cfg.partitions)
-      vary([&](std::vector<Knobs> &v) {
-        const bool hints = store_wants_hints(g, g.store.budget);
-        const int was = g.store_layout;
-        g.store_layout = best.layout;
-        const bool cur = store_sparse_first(g, hints);
-        for (int l : {0, 1, 2}) {
-          g.store_layout = l;
-          if (l != best.layout && store_sparse_first(g, hints) != cur) {
-            Knobs k = best;
-            k.layout = l;
-            v.push_back(k);
-            break;  // the other order: one candidate
-          }
-        }
-        g.store_layout = was;
       });
     if (!shape_only && cfg.partitions)
       vary([&](std::vector<Knobs> &v) {
