@@ -179,6 +179,12 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
  *                       parent hints; 1 densest first; 2 sparsest first with
  *                       the short-list side area.  Changing it rebuilds the
  *                       store with the same budget.
+ *   THV_OPT_DO_SUMMARY  a bottom-up level whose frontier holds fewer than
+ *                       n / value vertices (default 1024; 0 never), at
+ *                       the traversal's first probe, builds a frontier summary
+ *                       (one bit per 256 vertices) and its scans test it
+ *                       before reading a frontier word; a value below 1
+ *                       builds it at every bottom-up level.
  * BFS results are the same either way (levels are unique). */
 #define THV_OPT_SYMMETRIC 1
 #define THV_OPT_DIRECTION 2
@@ -194,6 +200,7 @@ int thv_graph_info(const thv_graph *g, uint64_t *n, uint64_t *m,
 #define THV_OPT_CC_UF_STEPS 12
 #define THV_OPT_KERNEL_TIMING 13
 #define THV_OPT_STORE_LAYOUT 14
+#define THV_OPT_DO_SUMMARY 15
 int thv_graph_set_option(thv_graph *g, int option, double value);
 int thv_graph_get_option(const thv_graph *g, int option, double *value);
 

@@ -161,6 +161,40 @@ def test_bfs_direction_optimizing(rmat16, frac):
     g.set_option("do_alpha", 14.0)
 
 
+@pytest.mark.parametrize("frac", [1.0, 0.25, 0.0])
+def test_bfs_frontier_summary(rmat16, frac):
+    """The bottom-up scans with a frontier summary at every level (do_summary
+    1e-9), and never (0): the oracle's levels and the same counts either
+    way; on the beads graph, whose bottom-up frontiers are one clique."""
+    g, og = rmat16
+    _say("case: frontier summary", frac)
+    g.store_config(int(frac * g.m * g.edge_bytes), 4096)
+    try:
+        bu = 0
+        for s in (1, 12345, 65535):
+            ref = O.bfs(og, s)
+            g.set_option("do_summary", 1e-9)
+            lv1, st1 = g.bfs(s)
+            g.set_option("do_summary", 0)
+            lv0, st0 = g.bfs(s)
+            assert np.array_equal(lv1, ref) and np.array_equal(lv0, ref), s
+            assert (st1.edges_traversed, st1.iterations, st1.bottom_up_levels) == \
+                (st0.edges_traversed, st0.iterations, st0.bottom_up_levels), s
+            bu += st1.bottom_up_levels
+        assert bu > 0
+    finally:
+        g.set_option("do_summary", 1024)
+    ob = _beads(40, 24)
+    gb = T.Graph.from_arrays(ob.off, ob.adj)
+    gb.set_option("symmetric", 1)
+    gb.set_option("do_alpha", 1000.0)
+    gb.set_option("do_beta", 1.0)
+    gb.set_option("do_summary", 1e-9)
+    gb.store_config(int(frac * ob.m * 4), 64)
+    for s in (0, 500):
+        assert np.array_equal(gb.bfs(s)[0], O.bfs(ob, s)), s
+
+
 @pytest.mark.parametrize("frac", [1.0, 0.25])
 def test_bfs_kernel_timing_off(rmat16, frac):
     """Without HIP events (kernel_timing 0) a direction-optimising BFS gives

@@ -29,7 +29,8 @@ PR_MODE = {"push": 0, "pull": 1}
 OPTIONS = {"symmetric": 1, "direction": 2, "do_alpha": 3, "do_beta": 4, "do_vertex_ns": 5,
            "do_link_gbps": 6, "do_pull_degree": 7, "parent_hints": 8, "hint_bytes": 9,
            "sssp_delta": 10, "cc_union": 11,
-           "cc_uf_steps": 12, "kernel_timing": 13, "store_layout": 14}
+           "cc_uf_steps": 12, "kernel_timing": 13, "store_layout": 14,
+           "do_summary": 15}
 ERRORS = {0: "OK", -1: "EINVAL", -2: "EIO", -3: "ENOMEM", -4: "EHIP", -5: "EFORMAT",
           -6: "EBUSY", -7: "ENOTSUP", -8: "ESTATE"}
 

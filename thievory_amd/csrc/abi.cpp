@@ -135,6 +135,10 @@ int thv_graph_set_option(thv_graph *gh, int option, double value) {
         g->do_pull_degree = (uint32_t)value;
         break;
       case THV_OPT_KERNEL_TIMING: g->kernel_timing = value != 0.0; break;
+      case THV_OPT_DO_SUMMARY:
+        NEED(value >= 0.0);
+        g->do_summary_ratio = value;
+        break;
       case THV_OPT_STORE_LAYOUT: {
         NEED(value == 0.0 || value == 1.0 || value == 2.0);
         const int old = g->store_layout;
@@ -167,6 +171,7 @@ int thv_graph_get_option(const thv_graph *gh, int option, double *value) {
       case THV_OPT_CC_UF_STEPS: *value = g->uf_max_steps; break;
       case THV_OPT_KERNEL_TIMING: *value = g->kernel_timing ? 1.0 : 0.0; break;
       case THV_OPT_STORE_LAYOUT: *value = g->store_layout; break;
+      case THV_OPT_DO_SUMMARY: *value = g->do_summary_ratio; break;
       default: fail(THV_EINVAL, "unknown option");
     }
   });

@@ -392,8 +392,9 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   // kernel reset it) publishes the long, short and next-frontier counters
   // (slots 2-4) and the probe counters; clear: the bitmap to clear, when no
   // count pass of this level did
+  // fsum: the frontier summary for the short-list scan, or null
   auto probe = [&](const uint32_t *front, const uint32_t *hint, uint32_t *next,
-                   uint32_t level_value, uint32_t *clear) {
+                   uint32_t level_value, uint32_t *clear, const uint32_t *fsum) {
     const int out = cand < 0 ? 0 : cand ^ 1;
     QueueSet &q = w.qs;
     LevelSide ps;
@@ -409,7 +410,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
                     timed ? w.ev[2] : nullptr, timed ? w.ev[3] : nullptr, c_next, w.val.as<uint32_t>(), level_value,
                     cand < 0 ? nullptr : q.clist[cand].as<uint32_t>(),
                     cand < 0 ? nullptr : q.ccnt[cand].as<uint32_t>(), q.clist[out].as<uint32_t>(),
-                    q.ccnt[out].as<uint32_t>(), &ps);
+                    q.ccnt[out].as<uint32_t>(), &ps, fsum);
     // a sweep reads the visited and residency words and writes the tile
     // tables of both sets, a list probe reads its regions' candidate
     // counts (the candidates themselves are counted on the device); both
@@ -576,6 +577,20 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     bool probed = false, hinted = false, sure = false;
     double td_cost = 0, bu_cost = 0;
     const uint32_t *hint = hints ? g.store.d_hint.as<uint32_t>() : nullptr;
+    // a frontier of under n / 1024 vertices (THV_OPT_DO_SUMMARY) leaves most
+    // summary bits clear: the bottom-up scans then test the summary before a
+    // frontier word -- at the traversal's first probe, which scans every
+    // unvisited vertex (a later probe's candidates are few by then)
+    const bool sparse_front = g.do_summary_ratio > 0 &&
+                              (double)nf * g.do_summary_ratio < (double)g.n &&
+                              (cand < 0 || g.do_summary_ratio < 1);
+    const uint32_t *fsum = nullptr;
+    auto summary = [&]() {
+      if (sparse_front && !fsum) {
+        launch_fsum(g, front, w.fsum.as<uint32_t>(), s);
+        fsum = w.fsum.as<uint32_t>();
+      }
+    };
     if (host_share) {
       // top-down streams the frontier's host records over the link and
       // writes and expands its queue in HBM; the hybrid bottom-up level
@@ -588,7 +603,8 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
           cand < 0 ? (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED : list_cost();
       bu = false;
       if (td_cost > probe_cost) {
-        probe(front, hint, next, L + 2, clear);
+        summary();
+        probe(front, hint, next, L + 2, clear, fsum);
         clear = nullptr;
         probed = true;
         hinted = hints;
@@ -636,7 +652,10 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       launch_expand(g, A_BFS, a, s);
       if (timed) HIPC(hipEventRecord(w.ev[1], s));
     } else {
-      if (!probed) probe(front, hint, next, L + 2, clear);
+      if (!probed) {
+        summary();
+        probe(front, hint, next, L + 2, clear, fsum);
+      }
       if (timed) HIPC(hipEventRecord(w.ev[0], s));
       // the probe scanned the short lists already (and, with hints, settled
       // every degree-1 vertex by its hint) and wrote the long ones, per wave
@@ -655,6 +674,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       b.rd_ctr = c_long;
       b.level_value = L + 2;
       b.nctr = c_next;
+      b.fsum = fsum;
       // published after it: slots 2-4 (records read, the short lists'
       // counts, the next frontier's sums); c_next is reset
       b.side.pub_src = c_long;

@@ -28,7 +28,7 @@ namespace {
 constexpr int BBS = 256;
 constexpr int G = 16;  // lanes per vertex
 
-template <int REC>
+template <int REC, bool SUM>
 __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
   __shared__ unsigned long long sh[BBS / 64];
   const uint32_t lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
@@ -75,7 +75,9 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
         bool hit = false;
         if (x < d) {
           const uint32_t w = *reinterpret_cast<const uint32_t *>(base + (unsigned long long)x * REC);
-          hit = (a.front[w >> 5] >> (w & 31)) & 1u;
+          // with a frontier summary, its bit first (an L2-resident word)
+          const bool maybe = !SUM || ((a.fsum[w >> (FSUM_SHIFT + 5)] >> ((w >> FSUM_SHIFT) & 31)) & 1u);
+          hit = maybe && ((a.front[w >> 5] >> (w & 31)) & 1u);
         }
         const unsigned long long b = __ballot(hit);
         if ((b >> (gw * G)) & 0xFFFFull) {
@@ -139,8 +141,13 @@ __global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *k
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
   if (a.side.zero) fail(THV_ESTATE, "the early-exit kernel clears no words");
   const int grid = g.shape.probe_bpc * g.cus;
-  if (g.rec == 8) hipLaunchKernelGGL(k_bottom_up<8>, dim3(grid), dim3(BBS), 0, s, a);
-  else hipLaunchKernelGGL(k_bottom_up<4>, dim3(grid), dim3(BBS), 0, s, a);
+  if (g.rec == 8) {
+    if (a.fsum) hipLaunchKernelGGL((k_bottom_up<8, true>), dim3(grid), dim3(BBS), 0, s, a);
+    else hipLaunchKernelGGL((k_bottom_up<8, false>), dim3(grid), dim3(BBS), 0, s, a);
+  } else {
+    if (a.fsum) hipLaunchKernelGGL((k_bottom_up<4, true>), dim3(grid), dim3(BBS), 0, s, a);
+    else hipLaunchKernelGGL((k_bottom_up<4, false>), dim3(grid), dim3(BBS), 0, s, a);
+  }
   if (a.side.pub || a.side.xpub || a.side.zctr)
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, a.side);
   HIPC(hipGetLastError());

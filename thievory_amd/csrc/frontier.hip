@@ -782,13 +782,15 @@ struct ShortArgs {
   uint32_t *val;
   uint32_t level_value;
   LevelCtr *nctr;
+  const uint32_t *fsum;     // frontier summary, nullable
 };
 
 // The short lists the probe collected: one wave per region (a wave's slots
 // of one probe sub-tile), one list per lane -- its address, its records and
-// their frontier words in three rounds of loads.  A list with a frontier
-// vertex places its vertex.
-template <int REC>
+// their frontier words in three rounds of loads (with a frontier summary,
+// its words first, and a frontier word only where the summary bit is set).
+// A list with a frontier vertex places its vertex.
+template <int REC, bool SUM>
 __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
   __shared__ unsigned long long sh[CBS / 64];
   const uint32_t lane = threadIdx.x & 63;
@@ -817,8 +819,21 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
       // their frontier words, loaded together (an absent record reads word
       // 0 and is ignored)
       uint32_t fw[SHORT_MAX];
+      if (SUM) {
+        uint32_t sw[SHORT_MAX];
 #pragma unroll
-      for (uint32_t k = 0; k < SHORT_MAX; k++) fw[k] = a.front[w[k] != 0xFFFFFFFFu ? w[k] >> 5 : 0];
+        for (uint32_t k = 0; k < SHORT_MAX; k++)
+          sw[k] = w[k] != 0xFFFFFFFFu ? a.fsum[w[k] >> (FSUM_SHIFT + 5)] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < SHORT_MAX; k++) {
+          fw[k] = 0u;
+          if ((sw[k] >> ((w[k] >> FSUM_SHIFT) & 31)) & 1u) fw[k] = a.front[w[k] >> 5];
+        }
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < SHORT_MAX; k++)
+          fw[k] = a.front[w[k] != 0xFFFFFFFFu ? w[k] >> 5 : 0];
+      }
       bool hit = false;
 #pragma unroll
       for (uint32_t k = 0; k < SHORT_MAX; k++)
@@ -886,7 +901,35 @@ __global__ void __launch_bounds__(CBS) k_touched(CompactArgs a) {
   }
 }
 
+// The frontier summary (launch_fsum): one lane per summary bit, over its 8
+// frontier words; lanes 0 and 32 write the wave's two summary words (the
+// bits past the last group read as zero).
+__global__ void __launch_bounds__(CBS) k_fsum(const uint32_t *front, uint64_t words,
+                                             uint32_t *fsum, uint64_t nbits) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t b = blockIdx.x * (uint64_t)CBS + threadIdx.x; b - lane < nbits;
+       b += (uint64_t)gridDim.x * CBS) {
+    uint32_t any = 0;
+    if (b < nbits) {
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++)
+        if (b * 8 + k < words) any |= front[b * 8 + k];
+    }
+    const unsigned long long m = __ballot(any != 0);
+    if (lane == 0) fsum[b >> 5] = (uint32_t)m;
+    if (lane == 32) fsum[b >> 5] = (uint32_t)(m >> 32);
+  }
+}
+
 }  // namespace
+
+void launch_fsum(Graph &g, const uint32_t *front, uint32_t *fsum, hipStream_t s) {
+  const uint64_t nbits = fsum_bits(g.n);
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(div_up(nbits, CBS), 8ull * g.cus));
+  hipLaunchKernelGGL(k_fsum, dim3(grid), dim3(CBS), 0, s, front, bitmap_words(g.n), fsum, nbits);
+  HIPC(hipGetLastError());
+}
 
 void launch_compact(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode, uint32_t *val, uint32_t val_value, uint32_t dmin,
@@ -953,7 +996,7 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
                      LevelCtr *cl, unsigned long long *cnt, hipStream_t s, hipEvent_t t0,
                      hipEvent_t t1, LevelCtr *nctr, uint32_t *val, uint32_t level_value,
                      const uint32_t *cin, const uint32_t *ccin, uint32_t *cout,
-                     uint32_t *ccout, const LevelSide *side) {
+                     uint32_t *ccout, const LevelSide *side, const uint32_t *fsum) {
   ProbeArgs p;
   memset(&p, 0, sizeof p);
   if (side && side->zctr) fail(THV_ESTATE, "the probe resets no counter");
@@ -1017,8 +1060,14 @@ void launch_bu_probe(Graph &g, QueueSet &w, const uint32_t *front, uint32_t *vis
     sa.val = val;
     sa.level_value = level_value;
     sa.nctr = nctr;
-    if (g.rec == 8) hipLaunchKernelGGL(k_bu_short<8>, dim3(grid), dim3(CBS), 0, s, sa);
-    else hipLaunchKernelGGL(k_bu_short<4>, dim3(grid), dim3(CBS), 0, s, sa);
+    sa.fsum = fsum;
+    if (g.rec == 8) {
+      if (fsum) hipLaunchKernelGGL((k_bu_short<8, true>), dim3(grid), dim3(CBS), 0, s, sa);
+      else hipLaunchKernelGGL((k_bu_short<8, false>), dim3(grid), dim3(CBS), 0, s, sa);
+    } else {
+      if (fsum) hipLaunchKernelGGL((k_bu_short<4, true>), dim3(grid), dim3(CBS), 0, s, sa);
+      else hipLaunchKernelGGL((k_bu_short<4, false>), dim3(grid), dim3(CBS), 0, s, sa);
+    }
   }
   if (t1) HIPC(hipEventRecord(t1, s));
   // tile bases and totals of both sets

@@ -466,6 +466,7 @@ void workspace_ensure(Graph &g) {
   const uint64_t words = bitmap_words(g.n);
   for (auto &f : w.front) f.alloc(words * 4);
   w.vis.alloc(words * 4);
+  w.fsum.alloc(fsum_words(g.n) * 4);
   w.aux.alloc(16 * sizeof(unsigned long long));
   w.val.alloc((g.n + 1) * 4);
   // fine-grained: a kernel's stores land in host memory without a cache

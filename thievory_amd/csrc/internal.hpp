@@ -160,6 +160,7 @@ struct Workspace {
   // other loops use [0] and [1].
   DevMem front[3];
   DevMem vis;                 // BFS visited bitmap
+  DevMem fsum;                // frontier summary (launch_fsum)
   DevMem val;                 // u32 values (BFS / CC / SSSP)
   DevMem f0, f1, f2;          // PR arrays (fp32)
   DevMem aux;                 // misc device counters (16 x u64)
@@ -220,6 +221,9 @@ struct Graph {
   bool cc_union = true;       // CC of a symmetric graph by union-find
   uint32_t uf_max_steps = 1u << 20;  // union-find step guard (then label propagation)
   bool kernel_timing = true;  // BFS: HIP events around its launches
+  // bottom-up levels scan with a frontier summary when frontier vertices x
+  // do_summary_ratio < n (0: never)
+  double do_summary_ratio = 1024.0;
   // store residency order: 0 chosen by the hints (sparsest first with them),
   // 1 densest partitions first, 2 sparsest first with the short-list side area
   int store_layout = 0;
@@ -336,7 +340,8 @@ void launch_bu_probe(Graph &g, QueueSet &qs, const uint32_t *front, uint32_t *vi
                      LevelCtr *nctr = nullptr, uint32_t *val = nullptr,
                      uint32_t level_value = 0, const uint32_t *cin = nullptr,
                      const uint32_t *ccin = nullptr, uint32_t *cout = nullptr,
-                     uint32_t *ccout = nullptr, const LevelSide *side = nullptr);
+                     uint32_t *ccout = nullptr, const LevelSide *side = nullptr,
+                     const uint32_t *fsum = nullptr);
 
 struct BottomUpArgs {
   const uint32_t *llist, *loff;  // the probe's long lists, per wave region,
@@ -350,11 +355,21 @@ struct BottomUpArgs {
   uint32_t level_value;
   LevelCtr *nctr;             // next-frontier bookkeeping (see launch_bu_probe)
   LevelSide side;             // published after the kernel (no zero words here)
+  const uint32_t *fsum = nullptr;  // frontier summary (launch_fsum), nullable
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
 void build_parent_hints(Graph &g);
 ExpandArgs expand_args(const Graph &g, const QueueSet &qs);
 inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
+// Frontier summary: bit j of fsum is set when vertices [256 j, 256 j + 256)
+// hold a frontier vertex (frontier words [8 j, 8 j + 8) not all zero).  A
+// bottom-up scan given it (the short-list scan, the early-exit kernel) reads
+// a frontier word only where the summary bit is set: for a small frontier
+// most records are settled by the summary, which stays in L2.
+constexpr uint32_t FSUM_SHIFT = 8;
+inline uint64_t fsum_bits(uint64_t n) { return (bitmap_words(n) + 7) / 8; }
+inline uint64_t fsum_words(uint64_t n) { return fsum_bits(n) / 32 + 3; }
+void launch_fsum(Graph &g, const uint32_t *front, uint32_t *fsum, hipStream_t s);
 void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s);
 
 // algos.hip
