@@ -287,8 +287,9 @@ static void check_src(Graph &g, uint32_t src) {
 }
 
 // Direction-optimising BFS on a symmetric graph (Beamer et al., SC'12).
-// Each level first counts the frontier (vertices n_f, edges m_f), unless
-// the last level ran bottom-up and summed it already.  All edges in HBM:
+// Each level knows its frontier's size (vertices n_f, edges m_f, per
+// residency): level 0 counts it, every later one reads the sums the last
+// level's kernels made while placing the vertices.  All edges in HBM:
 // with m_u the edges of still-unvisited vertices, the level runs bottom-up
 // when m_f > m_u / alpha and returns to top-down once n_f < n / beta and
 // the frontier shrinks; once a probe has left candidates, a level also
@@ -300,7 +301,9 @@ static void check_src(Graph &g, uint32_t src) {
 // unvisited set is counted and sorted into short and long lists), and the
 // cheaper direction in link time is taken (below).
 //
-// A top-down level is the ordinary compaction + expansion.  A bottom-up
+// A top-down level is a compaction (a full one: no count pass comes
+// before it) and an expansion that also writes the discovered vertices'
+// level and sums their degrees per residency (A_BFS_NEXT).  A bottom-up
 // level is the probe (a sweep of every vertex the first time, the previous
 // probe's candidates after that), the full scan of the short lists
 // (degree <= D, do_pull_degree) and the early-exit kernel over the long
@@ -311,9 +314,10 @@ static void check_src(Graph &g, uint32_t src) {
 // Host round trips: one stream sync per level (two when a probe's counts
 // decide the direction), and no memset or readback copy.  The kernel that
 // finishes a level's counters writes them to the host-mapped readback (the
-// count pass's scan, the probe's last launch, a one-wave launch after the
-// early-exit kernel); the count pass or the probe clears the next level's
-// frontier bitmap, one of three in rotation.
+// count pass's scan at level 0, the probe's last launch, a one-wave launch
+// after the early-exit kernel or the expansion); the count pass, the probe
+// or the compaction clears the next level's frontier bitmap, one of three
+// in rotation.
 constexpr int AUX_PC = 4;      // aux[4, 10): probe counters
 constexpr int HAUX_PC = 8;     // h_aux[8, 14): published probe counters
 
@@ -379,10 +383,31 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   double expand_ms = 0, bottom_up_ms = 0, probe_ms = 0, last_probe_ms = 0;
   uint32_t launches = 0, probes = 0;
   uint64_t probe_bytes = 0;  // the parts that do not come from d_pc
-  // a bottom-up level writes the values of the vertices it places and sums
-  // the next frontier (c_next), so the next level needs no count pass
+  // every level's kernels write the values of the vertices they place and
+  // sum the next frontier (c_next): a bottom-up level's probe, short-list
+  // and early-exit kernels, a top-down level's A_BFS_NEXT expansion; so only
+  // level 0 counts its frontier (fused_next: the next level's sums are in)
   LevelCtr *c_next = d_ctr + 4;
-  bool fused_next = false;
+  bool fused_next = false, probed_last = false;
+  // a top-down level's expansion, then its counters to the readback: slots
+  // 0-4, this level's totals with the partitions its write pass touched and
+  // the next frontier's sums (c_next, then reset)
+  auto expand_next = [&](LevelCtr *c, uint32_t level_value) {
+    a.ctr = c;
+    a.level_value = level_value;
+    a.deg = g.d_deg.as<uint32_t>();
+    a.rbit = g.store.d_rbit.as<uint32_t>();
+    a.nctr = c_next;
+    if (timed) HIPC(hipEventRecord(w.ev[0], s));
+    launch_expand(g, A_BFS_NEXT, a, s);
+    if (timed) HIPC(hipEventRecord(w.ev[1], s));
+    LevelSide ts;
+    ts.pub_src = d_ctr;
+    ts.pub = w.h_ctr_dev;
+    ts.npub = 5;
+    ts.zctr = c_next;
+    launch_publish(ts, s);
+  };
   // the first probe sweeps every vertex; it and every later one write the
   // vertices still unvisited after their hint test (candidates), and a
   // later probe reads only those (cand: the buffer holding the last list)
@@ -493,8 +518,8 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
         st->edges_examined += p.mf;
         st->host_bytes += p.h.qw[0] * rec;
         st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + 8 * words;
-        // k_touched added them after the count this level read; the next
-        // level's count pass published them with its own totals
+        // k_touched added them to this level's counters after their count;
+        // the publish launch after the expansion copied them out
         st->host_partitions_touched += w.h_ctr[p.L & 1].touched[0];
       } else {
         const uint64_t rh = hs.qw[0] + hb.rd[0], rd = hs.qw[1] + hb.rd[1];
@@ -517,12 +542,12 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     uint32_t *front = w.front[cur].as<uint32_t>();
     uint32_t *next = w.front[(cur + 1) % 3].as<uint32_t>();
     uint32_t *dead = w.front[(cur + 2) % 3].as<uint32_t>();  // the last level's frontier
-    // fused: the last level ran bottom-up, wrote this frontier's values and
-    // summed it (its early-exit kernel published the sums in slot 4);
-    // otherwise count it (and write its values) now.  The count pass
-    // publishes slots 0-1 (this level's totals and the partitions the last
-    // level's write pass touched), resets c_next and clears `dead` for the
-    // next level; a fused level has its probe or compaction clear it.
+    // fused (every level but 0): the last level wrote this frontier's values
+    // and summed it (published in slot 4); level 0 counts it (and writes its
+    // values) now.  The count pass publishes slots 0-1 (this level's totals
+    // and the partitions the last level's write pass touched), resets c_next
+    // and clears `dead` for the next level; a fused level has its probe or
+    // compaction clear it.
     const bool fused = fused_next;
     // Level 0 from a source whose whole list would not pay for a probe (or,
     // all edges in HBM, would not pass the edge-ratio rule) runs top-down
@@ -546,11 +571,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     }
     if (quick) {
       a.next = next;
-      a.ctr = c;
-      a.level_value = L + 2;
-      if (timed) HIPC(hipEventRecord(w.ev[0], s));
-      launch_expand(g, A_BFS, a, s);
-      if (timed) HIPC(hipEventRecord(w.ev[1], s));
+      expand_next(c, L + 2);
       visited_edges += d0;
       prev_nf = 1;
       pend = LevelRec();
@@ -558,7 +579,8 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       pend.L = L;
       pend.nf = 1;
       pend.mf = d0;
-      fused_next = false;
+      fused_next = true;
+      probed_last = false;
       cur = (cur + 1) % 3;
       continue;
     }
@@ -568,7 +590,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     const LevelCtr h = fused ? w.h_ctr[4] : w.h_ctr[L & 1];
     uint32_t *clear = fused ? dead : nullptr;
     const uint64_t nf = h.qn[0] + h.qn[1], mf = h.qw[0] + h.qw[1];
-    if (fused) last_cand = h.cand;
+    if (fused && probed_last) last_cand = h.cand;
     if (nf == 0) break;
     visited_edges += mf;
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
@@ -646,11 +668,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       zs.nzero = clear ? words : 0;
       launch_compact(g, w.qs, front, c, s, probed || fused ? 0 : CMP_WRITE_ONLY, nullptr, 0, 1,
                      0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &zs);
-      a.ctr = c;
-      a.level_value = L + 2;
-      if (timed) HIPC(hipEventRecord(w.ev[0], s));
-      launch_expand(g, A_BFS, a, s);
-      if (timed) HIPC(hipEventRecord(w.ev[1], s));
+      expand_next(c, L + 2);
     } else {
       if (!probed) {
         summary();
@@ -684,6 +702,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       launch_bottom_up(g, b, s);
       if (timed) HIPC(hipEventRecord(w.ev[1], s));
     }
+    pend = LevelRec();  // `later` belongs to the quick level 0 only
     pend.valid = true;
     pend.bu = bu;
     pend.probed = probed;
@@ -695,7 +714,8 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     pend.h = h;
     pend.td_cost = td_cost;
     pend.bu_cost = bu_cost;
-    fused_next = bu;
+    fused_next = true;
+    probed_last = probed;
     cur = (cur + 1) % 3;
   }
   if (st) {

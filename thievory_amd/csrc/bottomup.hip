@@ -138,6 +138,12 @@ __global__ void __launch_bounds__(BBS) k_hint_finish(const unsigned long long *k
 
 }  // namespace
 
+void launch_publish(const LevelSide &side, hipStream_t s) {
+  if (side.zero) fail(THV_ESTATE, "the publish launch clears no words");
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, side);
+  HIPC(hipGetLastError());
+}
+
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s) {
   if (a.side.zero) fail(THV_ESTATE, "the early-exit kernel clears no words");
   const int grid = g.shape.probe_bpc * g.cus;

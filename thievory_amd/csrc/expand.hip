@@ -66,6 +66,8 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   __shared__ uint32_t s_hk[HS];
   __shared__ float s_hv[HS];
 
+  // A_BFS_NEXT: the discovered vertices' entries and edges per residency
+  unsigned long long nfn[2] = {0, 0}, nfe[2] = {0, 0};
   const ChunkSrc cs = chunk_src(a.ctr, C);
   const uint32_t tid = threadIdx.x;
   if (ALGO == A_PR_PUSH) {
@@ -145,7 +147,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
     // edges before any result is used: the destination-state words (an
     // absent edge reads word 0 and is ignored), then the returning atomics
     // of the edges that need one, then the next-frontier bits.
-    if (ALGO == A_BFS) {
+    if (ALGO == A_BFS || ALGO == A_BFS_NEXT) {
       uint32_t vw[EPT], old[EPT];
 #pragma unroll
       for (int j = 0; j < EPT; j++) vw[j] = a.vis[dst[j] != 0xFFFFFFFFu ? dst[j] >> 5 : 0];
@@ -155,12 +157,32 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
         old[j] = 0xFFFFFFFFu;
         if (w != 0xFFFFFFFFu && !(vw[j] & bit)) old[j] = atomicOr(&a.vis[w >> 5], bit);
       }
-      // the winner only marks the next frontier; the next level's
-      // compaction writes the level of every frontier vertex
+      // the winner marks the next frontier; A_BFS leaves the level to the
+      // next level's compaction, A_BFS_NEXT writes it and sums the winners
+      // (their degree and residency words loaded together)
 #pragma unroll
       for (int j = 0; j < EPT; j++) {
         const uint32_t w = dst[j], bit = 1u << (w & 31);
         if (!(old[j] & bit)) atomicOr(&a.next[w >> 5], bit);
+      }
+      if (ALGO == A_BFS_NEXT) {
+        uint32_t dw[EPT], rw[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+          const uint32_t w = dst[j];
+          const bool won = !(old[j] & (1u << (w & 31)));
+          dw[j] = won ? a.deg[w] : 0u;
+          rw[j] = won ? a.rbit[w >> 5] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; j++) {
+          const uint32_t w = dst[j];
+          if (old[j] & (1u << (w & 31))) continue;
+          a.val[w] = a.level_value;
+          const uint32_t q = (rw[j] >> (w & 31)) & 1u;
+          nfn[q] += 1;
+          nfe[q] += dw[j];
+        }
       }
     } else if (ALGO == A_CC || ALGO == A_SSSP) {
       uint32_t cur[EPT], old[EPT], nv[EPT];
@@ -245,6 +267,17 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
     for (uint32_t t = tid; t < HS; t += EBS)
       if (s_hk[t] != 0xFFFFFFFFu && s_hv[t] != 0.0f) unsafeAtomicAdd(&a.pr_dst[s_hk[t]], s_hv[t]);
   }
+  if (ALGO == A_BFS_NEXT) {  // one atomic per counter and block
+    __shared__ unsigned long long s_n[EBS / 64];
+    const unsigned long long x0 = block_sum<EBS>(nfn[0], s_n), x1 = block_sum<EBS>(nfn[1], s_n);
+    const unsigned long long y0 = block_sum<EBS>(nfe[0], s_n), y1 = block_sum<EBS>(nfe[1], s_n);
+    if (tid == 0) {
+      if (x0) atomicAdd(&a.nctr->qn[0], x0);
+      if (x1) atomicAdd(&a.nctr->qn[1], x1);
+      if (y0) atomicAdd(&a.nctr->qw[0], y0);
+      if (y1) atomicAdd(&a.nctr->qw[1], y1);
+    }
+  }
 }
 
 template <int ALGO, int REC>
@@ -268,6 +301,11 @@ void launch_expand(Graph &g, Algo a, const ExpandArgs &args, hipStream_t s) {
   const bool r8 = g.rec == 8;
   switch (a) {
     case A_BFS: r8 ? launch_t<A_BFS, 8>(g, args, s) : launch_t<A_BFS, 4>(g, args, s); break;
+    case A_BFS_NEXT:
+      if (!args.nctr || !args.rbit || !args.deg || !args.val)
+        fail(THV_ESTATE, "next-frontier bookkeeping needs nctr, rbit, deg and val");
+      r8 ? launch_t<A_BFS_NEXT, 8>(g, args, s) : launch_t<A_BFS_NEXT, 4>(g, args, s);
+      break;
     case A_CC: r8 ? launch_t<A_CC, 8>(g, args, s) : launch_t<A_CC, 4>(g, args, s); break;
     case A_SSSP:
       if (!r8) fail(THV_ENOTSUP, "SSSP needs 8-byte {dst, weight} records");

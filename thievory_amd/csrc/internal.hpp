@@ -255,7 +255,8 @@ uint64_t now_ns();
 // (deg[w] << 32 | ~w).  A_CC_LINK: union-find link of every edge (val is
 // the parent array, err the step-guard flag).
 enum Algo {
-  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_HINT = 6, A_CC_LINK = 7
+  A_BFS = 0, A_CC = 1, A_SSSP = 2, A_PR_PUSH = 3, A_PR_PULL = 4, A_BFS_NEXT = 5, A_HINT = 6,
+  A_CC_LINK = 7
 };
 
 struct ExpandArgs {
@@ -279,6 +280,11 @@ struct ExpandArgs {
   uint32_t level_value;     // BFS: value given to discovered vertices
   uint32_t C;
   int rec;
+  // A_BFS_NEXT: BFS that also writes level_value for every vertex it
+  // discovers and sums them into nctr (entries and edges per residency, deg
+  // and rbit), so the next level needs no count pass
+  const uint32_t *rbit;
+  LevelCtr *nctr;
 };
 
 // Level bookkeeping a launch does on the side, so a level needs no memset
@@ -358,6 +364,9 @@ struct BottomUpArgs {
   const uint32_t *fsum = nullptr;  // frontier summary (launch_fsum), nullable
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
+// One wave copies side's counters to the host-mapped readback (no zero
+// words), after the launches before it on the stream.
+void launch_publish(const LevelSide &side, hipStream_t s);
 void build_parent_hints(Graph &g);
 ExpandArgs expand_args(const Graph &g, const QueueSet &qs);
 inline uint64_t bitmap_words(uint64_t n) { return n / 32 + 2; }
