@@ -225,6 +225,36 @@ def test_bfs_kernel_timing_off(rmat16, frac):
         g.set_option("kernel_timing", 1)
 
 
+@pytest.mark.parametrize("frac", [1.0, 0.25, 0.0])
+def test_bfs_values_after_other_runs(rmat16, frac):
+    """The value array is shared by every run on a graph: levels from a
+    source in a small component right after a BFS over the giant one, and
+    after CC (labels in the same array), are exact, top-down and
+    direction-optimising."""
+    g, og = rmat16
+    _say("case: bfs values after other runs", frac)
+    g.store_config(int(frac * g.m * g.edge_bytes), 4096)
+    lab = O.cc(og)
+    sizes = np.bincount(lab, minlength=og.n)
+    deg = og.degrees()
+    small = np.flatnonzero((sizes[lab] >= 2) & (sizes[lab] <= 8) & (deg > 0))
+    assert small.size > 0
+    v = int(small[0])
+    try:
+        for direction in (1, 0):
+            g.set_option("direction", direction)
+            for s in (1, v, 1, v):
+                assert np.array_equal(g.bfs(s)[0], O.bfs(og, s)), (direction, s)
+            assert np.array_equal(g.cc()[0], lab)
+            assert np.array_equal(g.bfs(v)[0], O.bfs(og, v)), (direction, v)
+            _, st = g.bfs(1)
+            assert np.array_equal(g.bfs(1)[0], O.bfs(og, 1))
+            if direction:
+                assert st.bottom_up_levels > 0
+    finally:
+        g.set_option("direction", 1)
+
+
 @pytest.mark.parametrize("layout", [1, 2])
 def test_store_layouts_exact(layout):
     """Both residency orders of a split store -- densest partitions first,

@@ -179,7 +179,9 @@ void build_parent_hints(Graph &g) {
   a.deg = g.d_deg.as<uint32_t>();
   a.key64 = key.as<unsigned long long>();
   launch_expand(g, A_HINT, a, s);
-  g.store.d_hint.alloc(g.n * 4);
+  g.store.d_hint.alloc(vertex_pad(g.n) * 4);
+  if (vertex_pad(g.n) > g.n)  // the padding: no hint
+    HIPC(hipMemsetAsync(g.store.d_hint.as<uint32_t>() + g.n, 0xFF, (vertex_pad(g.n) - g.n) * 4, s));
   const int grid = (int)std::min<uint64_t>(div_up(g.n, BBS), 16ull * g.cus);
   hipLaunchKernelGGL(k_hint_finish, dim3(grid ? grid : 1), dim3(BBS), 0, s,
                      key.as<unsigned long long>(), g.n, g.store.d_hint.as<uint32_t>());

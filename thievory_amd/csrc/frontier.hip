@@ -413,6 +413,7 @@ struct ProbeArgs {
 };
 
 constexpr uint32_t SHORT_MAX = 16;  // longest list the probe scans in full
+constexpr uint32_t PASS = ROUNDS / 4;  // the sweep's rounds per pass
 
 // The probes' counters: one atomic per counter and block (the grid is a
 // few blocks per CU).
@@ -448,12 +449,16 @@ __device__ __forceinline__ void probe_flush(const ProbeArgs &a, unsigned long lo
 }
 
 // Bottom-up probe, one wave per 1024-vertex sub-tile:
-//   1. visited and residency words (lanes 0-31, one load each);
-//   2. hints of the unvisited vertices and their frontier bits; a hit
-//      places the vertex (it is at level L+1 whichever way the level runs)
-//      and is written into the wave's own visited / next words by the lane
-//      that loaded them;
-//   3. degrees of the rest.  Lists of 1..D records go to the wave's slots
+//   1. visited, residency and next-frontier words (lanes 0-31, one load
+//      each);
+//   2. per pass of PASS rounds (64 vertices a round), hints and degrees
+//      loaded together, then the frontier words the unvisited vertices'
+//      hints name; a hit places the vertex (it is at level L+1 whichever
+//      way the level runs) and is written into the wave's own visited /
+//      next words by the lane that loaded them.  Passes keep a pass's
+//      loads in flight together within four waves' registers per SIMD
+//      (all 16 rounds at once spilled and ran no faster);
+//   3. the degrees of the rest.  Lists of 1..D records go to the wave's slots
 //      of the short-list array, in vertex order, for k_bu_short; longer
 //      ones to the long-list array, for the early-exit kernel; every
 //      unvisited vertex without a hint hit to the candidate list, which a
@@ -466,7 +471,8 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   __shared__ unsigned long long s[CBS / 64][10];
   const uint32_t lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned long long lt = lanemask_lt();
-  unsigned long long hits = 0, nun = 0, nhint = 0, nwords = 0, ncout = 0;
+  // per-lane counts (each at most n, which fits 32 bits; edge sums take 64)
+  uint32_t hits = 0, nun = 0, nhint = 0, nwords = 0, ncout = 0;
   unsigned long long fe[2] = {0, 0};  // next frontier: edges (host, HBM)
   uint32_t fn[2] = {0, 0};            //   and entries
   side_clear(a.side);
@@ -483,6 +489,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     const bool owner = lane < 32 && lane * 32 < nin;
     const uint32_t vw = owner ? visb[lane] : 0xFFFFFFFFu;
     const uint32_t rw = owner ? rbitb[lane] : 0u;
+    const uint32_t nw = owner ? nextb[lane] : 0u;  // this wave's own next words
     // fast path: no vertex of the block's tile is unvisited
     uint32_t uw = 0;
     if (owner) {
@@ -513,74 +520,88 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     }
     nun += __popc(un);
     uint32_t m = 0;  // hits of word `lane` (lanes 0-31)
-    uint32_t f = 0;  // bit r: round r's vertex is a hint hit
-    if (a.hint) {
-      const uint32_t *const hintb = a.hint + base;
-      uint32_t h[ROUNDS];
-#pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++)
-        h[r] = ((un >> r) & 1u) ? hintb[r * 64 + lane] : NO_HINT;
-#pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++) nhint += h[r] != NO_HINT;
-      // the frontier words the hints name, loaded together (a vertex
-      // without a hint reads word 0 and is ignored)
-      uint32_t fw[ROUNDS];
-#pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++) fw[r] = a.front[h[r] != NO_HINT ? h[r] >> 5 : 0];
-#pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++)
-        if (h[r] != NO_HINT && ((fw[r] >> (h[r] & 31)) & 1u)) f |= 1u << r;
-#pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++) {
-        const unsigned long long b = __ballot((f >> r) & 1u);
-        if (lane == 2 * r) m = (uint32_t)b;
-        if (lane == 2 * r + 1) m = (uint32_t)(b >> 32);
-      }
-      // with the bookkeeping on, the hits' degrees load with the others'
-      if (!a.nctr) un &= ~f;
-    }
-    uint32_t d[ROUNDS];
-#pragma unroll
-    for (uint32_t r = 0; r < ROUNDS; r++) d[r] = ((un >> r) & 1u) ? degb[r * 64 + lane] : 0u;
-    if (a.nctr && f) {
-#pragma unroll
-      for (uint32_t r = 0; r < ROUNDS; r++) {
-        if (!((f >> r) & 1u)) continue;
-        const uint32_t q = (dev >> r) & 1u;
-        (a.val + base)[r * 64 + lane] = a.level_value;
-        fn[q] += 1;
-        fe[q] += d[r];
-        d[r] = 0;  // placed: no list to scan
-      }
-      un &= ~f;
-    }
     // counts, and the wave's short lists (in vertex order).  With hints, a
     // degree-1 vertex is settled already: its hint is its one neighbour.
     const uint32_t dlo = a.hint ? 2u : 1u;
     uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
     unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
     uint32_t ns = 0, nl = 0, nc = 0;
+    // a pass loads its hints and degrees unconditionally (the lines are
+    // fetched whole anyway, and both arrays are padded to whole tiles), so
+    // that no load waits on a branch
+#pragma unroll 1
+    for (uint32_t r0 = 0; r0 < ROUNDS; r0 += PASS) {
+      uint32_t h[PASS], d[PASS];
+      if (a.hint) {
+        const uint32_t *const hintb = a.hint + base + r0 * 64;
 #pragma unroll
-    for (uint32_t r = 0; r < ROUNDS; r++) {
-      const uint32_t q = (dev >> r) & 1u;
-      const bool sh = d[r] >= dlo && d[r] <= a.D;
-      const bool lg = d[r] >= dlo && d[r] > a.D;
-      const unsigned long long b = __ballot(sh);  // every lane takes part
-      const unsigned long long bl = __ballot(lg);
-      if (sh) (a.slist + base)[ns + __popcll(b & lt)] = (uint32_t)base + r * 64 + lane;
-      if (lg) (a.llist + base)[nl + __popcll(bl & lt)] = (uint32_t)base + r * 64 + lane;
-      ns += __popcll(b);
-      nl += __popcll(bl);
-      if (a.cout) {
-        const bool c = (un >> r) & 1u;
-        const unsigned long long bc = __ballot(c);
-        if (c) (a.cout + base)[nc + __popcll(bc & lt)] = (uint32_t)base + r * 64 + lane;
-        nc += __popcll(bc);
+        for (uint32_t r = 0; r < PASS; r++) h[r] = hintb[r * 64 + lane];
       }
-      if (d[r] < dlo) continue;
-      const uint32_t k = (sh ? 0 : 2) + q;
-      xc[k] += 1;
-      xe[k] += d[r];
+      const uint32_t *const degh = degb + r0 * 64;
+#pragma unroll
+      for (uint32_t r = 0; r < PASS; r++) d[r] = degh[r * 64 + lane];
+      uint32_t f = 0;  // bit r: round r0 + r's vertex is a hint hit
+      if (a.hint) {
+#pragma unroll
+        for (uint32_t r = 0; r < PASS; r++) {
+          if (!((un >> (r0 + r)) & 1u)) h[r] = NO_HINT;
+          nhint += h[r] != NO_HINT;
+        }
+        // the frontier words the hints name, loaded together (a vertex
+        // without a hint reads word 0 and is ignored)
+        uint32_t fw[PASS];
+#pragma unroll
+        for (uint32_t r = 0; r < PASS; r++) fw[r] = a.front[h[r] != NO_HINT ? h[r] >> 5 : 0];
+#pragma unroll
+        for (uint32_t r = 0; r < PASS; r++)
+          if (h[r] != NO_HINT && ((fw[r] >> (h[r] & 31)) & 1u)) f |= 1u << r;
+#pragma unroll
+        for (uint32_t r = 0; r < PASS; r++) {
+          const unsigned long long b = __ballot((f >> r) & 1u);
+          if (lane == 2 * (r0 + r)) m = (uint32_t)b;
+          if (lane == 2 * (r0 + r) + 1) m = (uint32_t)(b >> 32);
+        }
+        // with the bookkeeping on, the hits keep their degrees for the sums
+        if (!a.nctr) un &= ~(f << r0);
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < PASS; r++)
+        if (!((un >> (r0 + r)) & 1u)) d[r] = 0;
+      if (a.nctr && f) {
+#pragma unroll
+        for (uint32_t r = 0; r < PASS; r++) {
+          if (!((f >> r) & 1u)) continue;
+          const uint32_t q = (dev >> (r0 + r)) & 1u;
+          (a.val + base + r0 * 64)[r * 64 + lane] = a.level_value;
+          fn[q] += 1;
+          fe[q] += d[r];
+          d[r] = 0;  // placed: no list to scan
+        }
+        un &= ~(f << r0);
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < PASS; r++) {
+        const uint32_t rr = r0 + r;
+        const uint32_t q = (dev >> rr) & 1u;
+        const bool sh = d[r] >= dlo && d[r] <= a.D;
+        const bool lg = d[r] >= dlo && d[r] > a.D;
+        const unsigned long long b = __ballot(sh);  // every lane takes part
+        const unsigned long long bl = __ballot(lg);
+        if (sh) (a.slist + base)[ns + __popcll(b & lt)] = (uint32_t)base + rr * 64 + lane;
+        if (lg) (a.llist + base)[nl + __popcll(bl & lt)] = (uint32_t)base + rr * 64 + lane;
+        ns += __popcll(b);
+        nl += __popcll(bl);
+        if (a.cout) {
+          const bool c = (un >> rr) & 1u;
+          const unsigned long long bc = __ballot(c);
+          if (c) (a.cout + base)[nc + __popcll(bc & lt)] = (uint32_t)base + rr * 64 + lane;
+          nc += __popcll(bc);
+        }
+        if (d[r] < dlo) continue;
+        const uint32_t k = (sh ? 0 : 2) + q;
+        xc[k] += 1;
+        xe[k] += d[r];
+      }
     }
     if (lane == 0) {
       a.scnt[tile * (CBS / 64) + wv] = ns;
@@ -593,7 +614,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     hits += __popc(m);  // hint hits
     if (owner && m) {
       visb[lane] = vw | m;
-      nextb[lane] |= m;
+      nextb[lane] = nw | m;
       nwords++;
     }
     // s[w][set * 4 + i]: i = 0, 1 entries (host, HBM), 2, 3 edges

@@ -109,15 +109,15 @@ void graph_init_device(Graph &g) {
 
 // Device degree tables: degrees and the degree-0 bitmap.
 void graph_build_tables(Graph &g) {
-  std::vector<uint32_t> deg(g.n), iso(bitmap_words(g.n), 0u);
+  std::vector<uint32_t> deg(vertex_pad(g.n), 0u), iso(bitmap_words(g.n), 0u);
   for (uint64_t v = 0; v < g.n; v++) {
     uint64_t d = g.h_off[v + 1] - g.h_off[v];
     if (d > 0xFFFFFFFFull) fail(THV_ENOTSUP, "vertex degree exceeds 2^32-1");
     deg[v] = (uint32_t)d;
     if (!d) iso[v >> 5] |= 1u << (v & 31);
   }
-  g.d_deg.alloc(g.n * 4);
-  if (g.n) HIPC(hipMemcpy(g.d_deg.p, deg.data(), g.n * 4, hipMemcpyHostToDevice));
+  g.d_deg.alloc(deg.size() * 4);
+  if (g.n) HIPC(hipMemcpy(g.d_deg.p, deg.data(), deg.size() * 4, hipMemcpyHostToDevice));
   g.d_iso.alloc(iso.size() * 4);
   HIPC(hipMemcpy(g.d_iso.p, iso.data(), iso.size() * 4, hipMemcpyHostToDevice));
 }

@@ -423,5 +423,9 @@ __host__ __device__ inline uint64_t mix64(uint64_t z) {
 }
 
 inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+// Entries of the degree and parent-hint arrays: n rounded up to whole tiles,
+// so the probe sweep loads every lane of a sub-tile unconditionally (the
+// padding holds degree 0 and no hint)
+inline uint64_t vertex_pad(uint64_t n) { return div_up(n, TILE) * TILE; }
 
 }  // namespace thv
