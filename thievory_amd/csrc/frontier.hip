@@ -664,6 +664,7 @@ __device__ __forceinline__ unsigned long long run_lanes(uint32_t o, uint32_t c, 
 // which owns the group's words), and the segment sums of both sets.  No
 // tile tables: after a probe nothing reads them.
 constexpr uint32_t LR = 8;  // regions per group
+constexpr uint32_t LU = 4;  // rounds of candidates whose loads are in flight together
 static_assert(SEG * (CBS / 64) % LR == 0, "a group lies in one segment");
 __global__ void __launch_bounds__(CBS) k_bu_probe_list(ProbeArgs a) {
   __shared__ uint32_t hw[CBS / 64][LR * (SUB / 32)];
@@ -698,64 +699,93 @@ __global__ void __launch_bounds__(CBS) k_bu_probe_list(ProbeArgs a) {
     uint32_t ns = 0, nl = 0, nc = 0;          // lane j: region r0 + j's counts so far
     uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
     unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
-    for (uint32_t t0 = 0; t0 < T; t0 += 64) {
-      const uint32_t t = t0 + lane;
-      // the lane's region: the last j whose run starts at or before t
-      uint32_t j = 0;
+    // LU rounds of 64 candidates at a time: their loads (candidate ids,
+    // then visited, residency, hint and degree words, then the frontier
+    // words the hints name) are in flight together; the lists are then
+    // written round by round, in candidate order
+    for (uint32_t tb = 0; tb < T; tb += 64 * LU) {
+      uint32_t v[LU], j[LU];
 #pragma unroll
-      for (uint32_t q = 1; q < LR; q++)
-        if ((uint32_t)__shfl((int)oj, q, 64) <= t) j = q;
-      const uint32_t oJ = (uint32_t)__shfl((int)oj, j, 64);
-      const uint32_t cJ = (uint32_t)__shfl((int)cj, j, 64);
-      const uint64_t rbase = (uint64_t)(r0 + j) * SUB;
-      const uint32_t v = t < T ? a.cin[rbase + (t - oJ)] : VNONE;
-      const uint32_t vw = v != VNONE ? a.vis[v >> 5] : 0xFFFFFFFFu;
-      const uint32_t rw = v != VNONE ? a.rbit[v >> 5] : 0u;
-      const uint32_t bit = 1u << (v & 31);
-      bool un = !(vw & bit);
-      const uint32_t q = (rw & bit) ? 1u : 0u;
-      nun += un;
-      bool hit = false;
-      if (a.hint) {
-        const uint32_t h = un ? a.hint[v] : NO_HINT;
+      for (uint32_t u = 0; u < LU; u++) {
+        const uint32_t t = tb + u * 64 + lane;
+        // the lane's region: the last j whose run starts at or before t
+        uint32_t jj = 0;
+#pragma unroll
+        for (uint32_t q = 1; q < LR; q++)
+          if ((uint32_t)__shfl((int)oj, q, 64) <= t) jj = q;
+        j[u] = jj;
+        const uint32_t oJ = (uint32_t)__shfl((int)oj, jj, 64);
+        v[u] = t < T ? a.cin[(uint64_t)(r0 + jj) * SUB + (t - oJ)] : VNONE;
+      }
+      // a lane past the candidates reads the words of vertex 0 (the hint
+      // and degree arrays are padded) and ignores them
+      uint32_t vw[LU], rw[LU], hr[LU], dr[LU];
+#pragma unroll
+      for (uint32_t u = 0; u < LU; u++) {
+        const uint32_t x = v[u] != VNONE ? v[u] : 0u;
+        vw[u] = a.vis[x >> 5];
+        rw[u] = a.rbit[x >> 5];
+        hr[u] = a.hint ? a.hint[x] : NO_HINT;
+        dr[u] = a.deg[x];
+      }
+      uint32_t fw[LU];
+#pragma unroll
+      for (uint32_t u = 0; u < LU; u++) {
+        if (v[u] == VNONE) vw[u] = 0xFFFFFFFFu;
+        const bool un = !(vw[u] & (1u << (v[u] & 31)));
+        if (!un) hr[u] = NO_HINT;
+        fw[u] = a.front[hr[u] != NO_HINT ? hr[u] >> 5 : 0];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < LU; u++) {
+        const uint32_t t0 = tb + u * 64;
+        if (t0 >= T) break;  // wave-uniform
+        const uint32_t vv = v[u], jj = j[u];
+        const uint32_t oJ = (uint32_t)__shfl((int)oj, jj, 64);
+        const uint32_t cJ = (uint32_t)__shfl((int)cj, jj, 64);
+        const uint64_t rbase = (uint64_t)(r0 + jj) * SUB;
+        const uint32_t bit = 1u << (vv & 31);
+        bool un = !(vw[u] & bit);
+        const uint32_t q = (rw[u] & bit) ? 1u : 0u;
+        nun += un;
+        const uint32_t h = hr[u];
         nhint += h != NO_HINT;
-        const uint32_t fw = a.front[h != NO_HINT ? h >> 5 : 0];
-        hit = h != NO_HINT && ((fw >> (h & 31)) & 1u);
+        const bool hit = h != NO_HINT && ((fw[u] >> (h & 31)) & 1u);
         if (hit) {
-          const uint32_t o = (uint32_t)(v - rbase);
-          atomicOr(&hw[wv][j * (SUB / 32) + (o >> 5)], bit);
+          const uint32_t o = (uint32_t)(vv - rbase);
+          atomicOr(&hw[wv][jj * (SUB / 32) + (o >> 5)], bit);
         }
-      }
-      // with the bookkeeping on, a hit's degree is the next frontier's
-      uint32_t d = un && (!hit || a.nctr) ? a.deg[v] : 0u;
-      if (hit) {
-        if (a.nctr) {
-          a.val[v] = a.level_value;
-          fn[q] += 1;
-          fe[q] += d;
+        // with the bookkeeping on, a hit's degree is the next frontier's
+        uint32_t d = un && (!hit || a.nctr) ? dr[u] : 0u;
+        if (hit) {
+          if (a.nctr) {
+            a.val[vv] = a.level_value;
+            fn[q] += 1;
+            fe[q] += d;
+          }
+          d = 0;
+          un = false;
         }
-        d = 0;
-        un = false;
-      }
-      const bool sh = un && d >= dlo && d <= a.D;
-      const bool lg = un && d >= dlo && d > a.D;
-      const unsigned long long mj = run_lanes(oJ, cJ, t0);
-      const unsigned long long bs = __ballot(sh), bl = __ballot(lg), bc = __ballot(un);
-      const uint32_t ps = (uint32_t)__shfl((int)ns, j, 64), pl = (uint32_t)__shfl((int)nl, j, 64);
-      const uint32_t pc = (uint32_t)__shfl((int)nc, j, 64);
-      if (sh) a.slist[rbase + ps + __popcll(bs & lt & mj)] = v;
-      if (lg) a.llist[rbase + pl + __popcll(bl & lt & mj)] = v;
-      if (un && a.cout) a.cout[rbase + pc + __popcll(bc & lt & mj)] = v;
-      if (lane < LR) {  // this round's lanes of region r0 + lane
-        const unsigned long long mo = run_lanes(oj, cj, t0);
-        ns += __popcll(bs & mo);
-        nl += __popcll(bl & mo);
-        nc += __popcll(bc & mo);
-      }
-      if (un && d >= dlo) {
-        const uint32_t k = (sh ? 0 : 2) + q;
-        xc[k] += 1;
-        xe[k] += d;
+        const bool sh = un && d >= dlo && d <= a.D;
+        const bool lg = un && d >= dlo && d > a.D;
+        const unsigned long long mj = run_lanes(oJ, cJ, t0);
+        const unsigned long long bs = __ballot(sh), bl = __ballot(lg), bc = __ballot(un);
+        const uint32_t ps = (uint32_t)__shfl((int)ns, jj, 64), pl = (uint32_t)__shfl((int)nl, jj, 64);
+        const uint32_t pc = (uint32_t)__shfl((int)nc, jj, 64);
+        if (sh) a.slist[rbase + ps + __popcll(bs & lt & mj)] = vv;
+        if (lg) a.llist[rbase + pl + __popcll(bl & lt & mj)] = vv;
+        if (un && a.cout) a.cout[rbase + pc + __popcll(bc & lt & mj)] = vv;
+        if (lane < LR) {  // this round's lanes of region r0 + lane
+          const unsigned long long mo = run_lanes(oj, cj, t0);
+          ns += __popcll(bs & mo);
+          nl += __popcll(bl & mo);
+          nc += __popcll(bc & mo);
+        }
+        if (un && d >= dlo) {
+          const uint32_t k = (sh ? 0 : 2) + q;
+          xc[k] += 1;
+          xe[k] += d;
+        }
       }
     }
     if (rl) {
