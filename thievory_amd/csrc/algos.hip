@@ -43,12 +43,12 @@ __global__ void k_set_bit(uint32_t *bm, uint32_t v) { bm[v >> 5] |= 1u << (v & 3
 
 // The start of a BFS from src in one launch: every value n + 1 but src's 1
 // (16-byte stores over the 256-byte aligned array); visited = the degree-0
-// vertices (iso, may be null) and src; frontier f0 = {src}; f1 clear; the
-// u64 counters z0[0, nz0) and z1[0, nz1) zero.
+// vertices (iso, may be null) and src; frontier f0 = {src}; f1 and f2 (may
+// be null) clear; the u64 counters z0[0, nz0) and z1[0, nz1) zero.
 __global__ void __launch_bounds__(256)
 k_bfs_init(uint32_t *val, uint64_t n, uint32_t src, uint32_t *vis, const uint32_t *iso,
-           uint32_t *f0, uint32_t *f1, uint64_t words, unsigned long long *z0, uint32_t nz0,
-           unsigned long long *z1, uint32_t nz1) {
+           uint32_t *f0, uint32_t *f1, uint32_t *f2, uint64_t words, unsigned long long *z0,
+           uint32_t nz0, unsigned long long *z1, uint32_t nz1) {
   const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
   const uint32_t far = (uint32_t)(n + 1);
@@ -72,6 +72,7 @@ k_bfs_init(uint32_t *val, uint64_t n, uint32_t src, uint32_t *vis, const uint32_
     vis[i] = (iso ? iso[i] : 0u) | b;
     f0[i] = b;
     f1[i] = 0u;
+    if (f2) f2[i] = 0u;
   }
   if (t < nz0) z0[t] = 0ull;
   if (t < nz1) z1[t] = 0ull;
@@ -459,6 +460,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   struct LevelRec {
     // sure: bottom-up without a readback of the probe's counts (see below);
     // later: level 0 read no counts, h comes from the readback at its end
+    // (and its queue was written without a pass over the bitmap)
     bool valid = false, bu = false, probed = false, hinted = false, sure = false, later = false;
     uint32_t L = 0;
     uint64_t nf = 0, mf = 0;
@@ -481,7 +483,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   double prev_bu_cost = bu_bound;
   auto finish_level = [&](LevelRec &p) {
     if (!p.valid) return;
-    if (p.later) p.h = w.h_ctr[p.L & 1];  // the next count pass republished it
+    if (p.later) p.h = w.h_ctr[p.L & 1];  // published after the expansion
     if (p.bu && (!p.probed || p.sure)) probe_done();
     if (p.sure) p.bu_cost = short_long_cost(w.h_ctr[2], w.h_ctr[3]);
     prev_bu_cost = p.bu && p.probed ? std::min(p.bu_cost, bu_bound) : bu_bound;
@@ -517,7 +519,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       if (!p.bu) {
         st->edges_examined += p.mf;
         st->host_bytes += p.h.qw[0] * rec;
-        st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + 8 * words;
+        st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + (p.later ? 0 : 8 * words);
         // k_touched added them to this level's counters after their count;
         // the publish launch after the expansion copied them out
         st->host_partitions_touched += w.h_ctr[p.L & 1].touched[0];
@@ -551,14 +553,19 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     const bool fused = fused_next;
     // Level 0 from a source whose whole list would not pay for a probe (or,
     // all edges in HBM, would not pass the edge-ratio rule) runs top-down
-    // whatever its counts: count and write passes back to back, no readback.
+    // whatever its counts: its one-entry queue and the expansion, no
+    // readback.
     const uint64_t d0 = L == 0 ? g.h_off[src + 1] - g.h_off[src] : 0;
     const bool quick =
         L == 0 && d0 > 0 &&
         (host_share ? (double)d0 * rec / link + td_hbm_cost(1, d0) <=
                           (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED
                     : (double)d0 <= (double)(g.m - std::min<uint64_t>(g.m, d0)) / g.do_alpha);
-    if (!fused) {
+    if (quick) {
+      // the frontier is {src}: its queue directly (the start cleared `dead`
+      // and c_next); the expansion's publish launch copies the counters out
+      launch_queue_one(g, w.qs, src, c, w.val.as<uint32_t>(), L + 1, s);
+    } else if (!fused) {
       LevelSide cs;
       cs.zero = dead;
       cs.nzero = words;
@@ -566,8 +573,8 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       cs.pub = w.h_ctr_dev;
       cs.npub = 2;
       cs.zctr = c_next;
-      launch_compact(g, w.qs, front, c, s, quick ? 0 : CMP_COUNT_ONLY, w.val.as<uint32_t>(),
-                     L + 1, 1, 0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &cs);
+      launch_compact(g, w.qs, front, c, s, CMP_COUNT_ONLY, w.val.as<uint32_t>(), L + 1, 1,
+                     0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &cs);
     }
     if (quick) {
       a.next = next;
@@ -755,7 +762,8 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   hipLaunchKernelGGL(k_bfs_init, dim3(fill_grid(std::max<uint64_t>(g.n / 4, words))), dim3(256),
                      0, s, w.val.as<uint32_t>(), g.n, src, w.vis.as<uint32_t>(),
                      dopt ? g.d_iso.as<uint32_t>() : nullptr, w.front[0].as<uint32_t>(),
-                     w.front[1].as<uint32_t>(), words,
+                     w.front[1].as<uint32_t>(), dopt ? w.front[2].as<uint32_t>() : nullptr,
+                     words,
                      dopt ? reinterpret_cast<unsigned long long *>(w.qs.ctr.as<LevelCtr>() + 4)
                           : nullptr,
                      dopt ? (uint32_t)(sizeof(LevelCtr) / 8) : 0u,

@@ -321,6 +321,12 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 // *minout to the smallest fval of a selected vertex.  side (count pass
 // only; ignored by WRITE_ONLY): the count kernel clears side->zero, the
 // scan publishes after writing the totals and resets side->zctr.
+// The queue of a one-vertex frontier {v} (degree >= 1) without a pass over
+// the bitmap: what launch_compact would write for it -- the entry, its edge
+// offset and chunk starts, and ctr's totals with v's partition as the one
+// touched (ctr's cand kept) -- and val[v] = val_value.  One small launch.
+void launch_queue_one(Graph &g, QueueSet &qs, uint32_t v, LevelCtr *ctr, uint32_t *val,
+                      uint32_t val_value, hipStream_t s);
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode = 0, uint32_t *val = nullptr,
                     uint32_t val_value = 0, uint32_t dmin = 1,
