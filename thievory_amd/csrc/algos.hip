@@ -460,8 +460,10 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   struct LevelRec {
     // sure: bottom-up without a readback of the probe's counts (see below);
     // later: level 0 read no counts, h comes from the readback at its end
-    // (and its queue was written without a pass over the bitmap)
-    bool valid = false, bu = false, probed = false, hinted = false, sure = false, later = false;
+    // (and its queue was written without a pass over the bitmap); sparse: a
+    // top-down level whose queue came from one pass listing its vertices
+    bool valid = false, bu = false, probed = false, hinted = false, sure = false, later = false,
+         sparse = false;
     uint32_t L = 0;
     uint64_t nf = 0, mf = 0;
     LevelCtr h;
@@ -519,7 +521,8 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       if (!p.bu) {
         st->edges_examined += p.mf;
         st->host_bytes += p.h.qw[0] * rec;
-        st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf + (p.later ? 0 : 8 * words);
+        st->hbm_bytes += p.h.qw[1] * rec + 4 * p.mf + 12 * p.nf +
+                         (p.later ? 0 : p.sparse ? 4 * words : 8 * words);
         // k_touched added them to this level's counters after their count;
         // the publish launch after the expansion copied them out
         st->host_partitions_touched += w.h_ctr[p.L & 1].touched[0];
@@ -603,7 +606,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     const uint64_t mu = g.m - std::min<uint64_t>(g.m, visited_edges);
     // probed: the bottom-up probe ran (tile set 0 now holds the short
     // unvisited lists, so a top-down level needs a fresh frontier count)
-    bool probed = false, hinted = false, sure = false;
+    bool probed = false, hinted = false, sure = false, sparse = false;
     double td_cost = 0, bu_cost = 0;
     const uint32_t *hint = hints ? g.store.d_hint.as<uint32_t>() : nullptr;
     // a frontier of under n / 1024 vertices (THV_OPT_DO_SUMMARY) leaves most
@@ -673,8 +676,14 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
       LevelSide zs;
       zs.zero = clear;
       zs.nzero = clear ? words : 0;
-      launch_compact(g, w.qs, front, c, s, probed || fused ? 0 : CMP_WRITE_ONLY, nullptr, 0, 1,
-                     0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &zs);
+      if (nf <= SPARSE_MAX) {
+        // a small frontier: listed by one pass over the bitmap, no tile tables
+        launch_queue_sparse(g, w.qs, front, c, s, &zs);
+        sparse = true;
+      } else {
+        launch_compact(g, w.qs, front, c, s, probed || fused ? 0 : CMP_WRITE_ONLY, nullptr, 0,
+                       1, 0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &zs);
+      }
       expand_next(c, L + 2);
     } else {
       if (!probed) {
@@ -715,6 +724,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     pend.probed = probed;
     pend.hinted = hinted;
     pend.sure = sure;
+    pend.sparse = sparse;
     pend.L = L;
     pend.nf = nf;
     pend.mf = mf;
@@ -756,17 +766,18 @@ void run_bfs(Graph &g, uint32_t src, uint32_t *out, thv_stats *st) {
   stats_begin(st);
   // on a symmetric graph a degree-0 vertex has no in-edges either: it
   // starts visited (its level stays n+1), so bottom-up levels never look at
-  // it.  The direction-optimising loop also starts from zero next-frontier
-  // sums (ctr slot 4) and probe counters (aux[4, 10)).
+  // it.  The direction-optimising loop also starts from zero counter slots
+  // (the next-frontier sums in slot 4, the sparse-queue list counts) and
+  // probe counters (aux[4, 10)).
   const bool dopt = g.sym && g.dir_opt;
   hipLaunchKernelGGL(k_bfs_init, dim3(fill_grid(std::max<uint64_t>(g.n / 4, words))), dim3(256),
                      0, s, w.val.as<uint32_t>(), g.n, src, w.vis.as<uint32_t>(),
                      dopt ? g.d_iso.as<uint32_t>() : nullptr, w.front[0].as<uint32_t>(),
                      w.front[1].as<uint32_t>(), dopt ? w.front[2].as<uint32_t>() : nullptr,
                      words,
-                     dopt ? reinterpret_cast<unsigned long long *>(w.qs.ctr.as<LevelCtr>() + 4)
+                     dopt ? reinterpret_cast<unsigned long long *>(w.qs.ctr.as<LevelCtr>())
                           : nullptr,
-                     dopt ? (uint32_t)(sizeof(LevelCtr) / 8) : 0u,
+                     dopt ? (uint32_t)(CTR_SLOTS * sizeof(LevelCtr) / 8) : 0u,
                      dopt ? w.aux.as<unsigned long long>() + AUX_PC : nullptr,
                      dopt ? 6u : 0u);
   HIPC(hipGetLastError());

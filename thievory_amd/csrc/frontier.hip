@@ -975,6 +975,100 @@ __global__ void __launch_bounds__(CBS) k_queue_one(CompactArgs a, uint32_t v) {
   for (uint64_t c = threadIdx.x; c * a.C < d; c += CBS) a.qc[q][c] = 0;
 }
 
+// A sparse frontier's queue (launch_queue_sparse).  k_list_front: one pass
+// over the bitmap, four words per thread, appends the frontier vertices to
+// `list` (one atomic per wave that holds any; the caller bounds the
+// frontier, so few waves do) and clears side.zero as side work.
+__global__ void __launch_bounds__(CBS) k_list_front(const uint32_t *front, uint64_t words,
+                                                    uint32_t *list, uint32_t cap,
+                                                    unsigned int *count, LevelSide side) {
+  side_clear(side);
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t b = (uint64_t)blockIdx.x * CBS; b * 4 < words; b += (uint64_t)gridDim.x * CBS) {
+    const uint64_t i = b + threadIdx.x;  // words [4i, 4i + 4)
+    uint32_t x[4] = {0u, 0u, 0u, 0u};
+    if (4 * i + 3 < words) {
+      const uint4 q = reinterpret_cast<const uint4 *>(front)[i];
+      x[0] = q.x;
+      x[1] = q.y;
+      x[2] = q.z;
+      x[3] = q.w;
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++)
+        if (4 * i + k < words) x[k] = front[4 * i + k];
+    }
+    const uint32_t c = __popc(x[0]) + __popc(x[1]) + __popc(x[2]) + __popc(x[3]);
+    const uint32_t inc = wave_incl_scan(c);
+    const uint32_t tot = (uint32_t)__shfl((int)inc, 63, 64);
+    if (tot == 0) continue;  // wave-uniform
+    uint32_t o = 0;
+    if (lane == 0) o = atomicAdd(count, tot);
+    o = (uint32_t)__shfl((int)o, 0, 64) + inc - c;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      for (uint32_t wd = x[k]; wd; wd &= wd - 1, o++)
+        if (o < cap) list[o] = (uint32_t)((4 * i + k) * 32 + (uint32_t)(__ffs(wd) - 1));
+    }
+  }
+}
+
+// k_queue_list: one block writes the queue of the listed vertices (with
+// dmin <= degree <= dmax), in list order: entries and edge offsets per
+// residency from block scans, the chunk starts, and the totals with the
+// partitions touched (the first to stamp a partition counts it); ctr's cand
+// is kept and the list count reset.
+constexpr int QBS = 1024;
+__global__ void __launch_bounds__(QBS) k_queue_list(CompactArgs a, const uint32_t *list,
+                                                    uint32_t cap) {
+  __shared__ unsigned long long sh[6][QBS / 64 + 1];
+  unsigned int *count = &a.ctr->pad[0];
+  const uint32_t cnt = min(*count, cap);
+  const uint32_t per = (cnt + QBS - 1) / QBS;
+  const uint32_t i0 = min(threadIdx.x * per, cnt), i1 = min(i0 + per, cnt);
+  unsigned long long n[2] = {0, 0}, e[2] = {0, 0};
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t v = list[i], d = a.deg[v];
+    if (d < a.dmin || d > a.dmax) continue;
+    const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
+    n[q] += 1;
+    e[q] += d;
+  }
+  unsigned long long tn[2], te[2], idx[2], eb[2];
+  idx[0] = block_excl_scan<QBS>(n[0], sh[0], &tn[0]);
+  idx[1] = block_excl_scan<QBS>(n[1], sh[1], &tn[1]);
+  eb[0] = block_excl_scan<QBS>(e[0], sh[2], &te[0]);
+  eb[1] = block_excl_scan<QBS>(e[1], sh[3], &te[1]);
+  unsigned long long th[2] = {0, 0};
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t v = list[i], d = a.deg[v];
+    if (d < a.dmin || d > a.dmax) continue;
+    const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
+    const unsigned long long k = idx[q]++, e0 = eb[q];
+    eb[q] += d;
+    a.qv[q][k] = v;
+    a.qe[q][k] = e0;
+    for (unsigned long long c = (e0 + a.C - 1) / a.C; c * a.C < e0 + d; c++)
+      a.qc[q][c] = (uint32_t)k;
+    const uint32_t p = a.vpart[v];
+    if (atomicExch(&a.ptouch[p], a.stamp) != a.stamp) th[a.pres[p] ? Q_DEV : Q_HOST]++;
+  }
+  const unsigned long long t0 = block_sum<QBS>(th[0], sh[4]);
+  const unsigned long long t1 = block_sum<QBS>(th[1], sh[5]);
+  if (threadIdx.x == 0) {
+    LevelCtr t = *a.ctr;  // keeps cand
+    t.qn[0] = tn[0];
+    t.qn[1] = tn[1];
+    t.qw[0] = te[0];
+    t.qw[1] = te[1];
+    t.touched[0] = (unsigned)t0;
+    t.touched[1] = (unsigned)t1;
+    t.rd[0] = t.rd[1] = 0;
+    t.pad[0] = 0;  // the list count, for the next sparse level
+    *a.ctr = t;
+  }
+}
+
 // The frontier summary (launch_fsum): one lane per summary bit, over its 8
 // frontier words; lanes 0 and 32 write the wave's two summary words (the
 // bits past the last group read as zero).
@@ -1025,6 +1119,43 @@ void launch_queue_one(Graph &g, QueueSet &w, uint32_t v, LevelCtr *ctr, uint32_t
   a.val = val;
   a.val_value = val_value;
   hipLaunchKernelGGL(k_queue_one, dim3(1), dim3(CBS), 0, s, a, v);
+  HIPC(hipGetLastError());
+}
+
+void launch_queue_sparse(Graph &g, QueueSet &w, const uint32_t *front, LevelCtr *ctr,
+                         hipStream_t s, const LevelSide *side) {
+  if (side && (side->pub || side->xpub || side->zctr))
+    fail(THV_ESTATE, "the sparse queue only clears words");
+  CompactArgs a;
+  memset(&a, 0, sizeof a);
+  a.rbit = g.store.d_rbit.as<uint32_t>();
+  a.deg = g.d_deg.as<uint32_t>();
+  a.vpart = g.store.d_vpart.as<uint32_t>();
+  a.pres = g.store.d_pres.as<uint8_t>();
+  for (int q = 0; q < 2; q++) {
+    a.qv[q] = w.qv[q].as<uint32_t>();
+    a.qe[q] = w.qe[q].as<unsigned long long>();
+    a.qc[q] = w.qc[q].as<uint32_t>();
+  }
+  a.ptouch = w.ptouch.as<uint32_t>();
+  a.stamp = ++w.stamp;
+  if (a.stamp == 0) {  // wrapped: old stamps could collide
+    HIPC(hipMemsetAsync(w.ptouch.p, 0, (uint64_t)(w.P + 1) * 4, s));
+    a.stamp = w.stamp = 1;
+  }
+  a.ctr = ctr;
+  a.C = w.C;
+  a.dmin = 1;
+  a.dmax = 0xFFFFFFFFu;
+  LevelSide zs;
+  if (side) zs = *side;
+  const uint64_t words = bitmap_words(g.n);
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(div_up(words, 4 * CBS), 4ull * g.cus));
+  hipLaunchKernelGGL(k_list_front, dim3(grid), dim3(CBS), 0, s, front, words,
+                     w.slist.as<uint32_t>(), SPARSE_MAX, &ctr->pad[0], zs);
+  hipLaunchKernelGGL(k_queue_list, dim3(1), dim3(QBS), 0, s, a, w.slist.as<const uint32_t>(),
+                     SPARSE_MAX);
   HIPC(hipGetLastError());
 }
 

@@ -327,6 +327,14 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 // touched (ctr's cand kept) -- and val[v] = val_value.  One small launch.
 void launch_queue_one(Graph &g, QueueSet &qs, uint32_t v, LevelCtr *ctr, uint32_t *val,
                       uint32_t val_value, hipStream_t s);
+// The queue of a frontier of at most SPARSE_MAX vertices (the caller knows
+// its size) without tile tables: one pass over the bitmap lists its
+// vertices (into qs.slist, counted in ctr->pad[0], which must be 0 and is
+// left 0) and one block writes the queue and ctr's totals as launch_compact
+// would, with the touched partitions; side (may be null): words to clear.
+constexpr uint32_t SPARSE_MAX = 4096;
+void launch_queue_sparse(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
+                         hipStream_t s, const LevelSide *side);
 void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                     hipStream_t s, int mode = 0, uint32_t *val = nullptr,
                     uint32_t val_value = 0, uint32_t dmin = 1,
