@@ -841,6 +841,13 @@ struct ShortArgs {
 // their frontier words in three rounds of loads (with a frontier summary,
 // its words first, and a frontier word only where the summary bit is set).
 // A list with a frontier vertex places its vertex.
+// The first word of the record at a store address (HBM or mapped host
+// memory) as a global-memory load: a generic (flat) load also counts
+// against the LDS counter, whose waits drain every load in flight.
+__device__ __forceinline__ uint32_t ld_record(unsigned long long addr) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(addr);
+}
+
 template <int REC, bool SUM>
 __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
   __shared__ unsigned long long sh[CBS / 64];
@@ -851,8 +858,12 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
        rg += nwaves) {
     const uint32_t cnt = a.scnt[rg];
     const uint64_t base = (uint64_t)rg * SUB;
+    // the lane's next list id loads one step ahead (the last id again at
+    // the end of the region)
+    uint32_t vn = cnt ? a.slist[base + min(lane, cnt - 1)] : 0u;
     for (uint32_t i = lane; i < cnt; i += 64) {
-      const uint32_t v = a.slist[base + i];
+      const uint32_t v = vn;
+      vn = a.slist[base + min(i + 64, cnt - 1)];
       const uint32_t dl = a.deg[v];  // 1..16
       const unsigned long long eb = a.eptr[v];
       // the first EAGER records load unconditionally (the index clamped to
@@ -861,12 +872,13 @@ __global__ void __launch_bounds__(CBS) k_bu_short(ShortArgs a) {
       constexpr uint32_t EAGER = 4;
       uint32_t w[SHORT_MAX];
 #pragma unroll
-      for (uint32_t k = 0; k < EAGER; k++)
-        w[k] = *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)min(k, dl - 1) * REC);
+      for (uint32_t k = 0; k < EAGER; k++) {
+        const uint32_t x = ld_record(eb + (unsigned long long)min(k, dl - 1) * REC);
+        w[k] = k < dl ? x : 0xFFFFFFFFu;
+      }
 #pragma unroll
       for (uint32_t k = EAGER; k < SHORT_MAX; k++)
-        w[k] = k < dl ? *reinterpret_cast<const uint32_t *>(eb + (unsigned long long)k * REC)
-                      : 0xFFFFFFFFu;
+        w[k] = k < dl ? ld_record(eb + (unsigned long long)k * REC) : 0xFFFFFFFFu;
       // their frontier words, loaded together (an absent record reads word
       // 0 and is ignored)
       uint32_t fw[SHORT_MAX];
