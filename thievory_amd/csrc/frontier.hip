@@ -470,7 +470,6 @@ __device__ __forceinline__ void probe_flush(const ProbeArgs &a, unsigned long lo
 __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
   __shared__ unsigned long long s[CBS / 64][10];
   const uint32_t lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const unsigned long long lt = lanemask_lt();
   // per-lane counts (each at most n, which fits 32 bits; edge sums take 64)
   uint32_t hits = 0, nun = 0, nhint = 0, nwords = 0, ncout = 0;
   unsigned long long fe[2] = {0, 0};  // next frontier: edges (host, HBM)
@@ -523,8 +522,10 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
     // counts, and the wave's short lists (in vertex order).  With hints, a
     // degree-1 vertex is settled already: its hint is its one neighbour.
     const uint32_t dlo = a.hint ? 2u : 1u;
-    uint32_t xc[4] = {0, 0, 0, 0};            // entries: short host/HBM, long host/HBM
-    unsigned long long xe[4] = {0, 0, 0, 0};  // edges, same order
+    // entries: short host/HBM, long host/HBM (wave-uniform); the short
+    // lists' records, host/HBM, per lane (at most 16 per vertex).  The long
+    // lists' records are not summed: nothing reads them.
+    uint32_t xc[4] = {0, 0, 0, 0}, es[2] = {0, 0};
     uint32_t ns = 0, nl = 0, nc = 0;
     // a pass loads its hints and degrees unconditionally (the lines are
     // fetched whole anyway, and both arrays are padded to whole tiles), so
@@ -557,7 +558,7 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
           if (h[r] != NO_HINT && ((fw[r] >> (h[r] & 31)) & 1u)) f |= 1u << r;
 #pragma unroll
         for (uint32_t r = 0; r < PASS; r++) {
-          const unsigned long long b = __ballot((f >> r) & 1u);
+          const unsigned long long b = wave_ballot((f >> r) & 1u);
           if (lane == 2 * (r0 + r)) m = (uint32_t)b;
           if (lane == 2 * (r0 + r) + 1) m = (uint32_t)(b >> 32);
         }
@@ -579,28 +580,34 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
         }
         un &= ~(f << r0);
       }
+      // the lists in vertex order; the class counts from the ballots (wave-
+      // uniform, on the scalar unit), the short lists' records per lane
 #pragma unroll
       for (uint32_t r = 0; r < PASS; r++) {
         const uint32_t rr = r0 + r;
-        const uint32_t q = (dev >> rr) & 1u;
+        const bool q = (dev >> rr) & 1u;
         const bool sh = d[r] >= dlo && d[r] <= a.D;
         const bool lg = d[r] >= dlo && d[r] > a.D;
-        const unsigned long long b = __ballot(sh);  // every lane takes part
-        const unsigned long long bl = __ballot(lg);
-        if (sh) (a.slist + base)[ns + __popcll(b & lt)] = (uint32_t)base + rr * 64 + lane;
-        if (lg) (a.llist + base)[nl + __popcll(bl & lt)] = (uint32_t)base + rr * 64 + lane;
+        const unsigned long long b = wave_ballot(sh);  // every lane takes part
+        const unsigned long long bl = wave_ballot(lg);
+        const unsigned long long bq = wave_ballot(q);
+        const uint32_t v = (uint32_t)base + rr * 64 + lane;
+        if (sh) (a.slist + base)[ns + lanes_below(b)] = v;
+        if (lg) (a.llist + base)[nl + lanes_below(bl)] = v;
         ns += __popcll(b);
         nl += __popcll(bl);
         if (a.cout) {
           const bool c = (un >> rr) & 1u;
-          const unsigned long long bc = __ballot(c);
-          if (c) (a.cout + base)[nc + __popcll(bc & lt)] = (uint32_t)base + rr * 64 + lane;
+          const unsigned long long bc = wave_ballot(c);
+          if (c) (a.cout + base)[nc + lanes_below(bc)] = v;
           nc += __popcll(bc);
         }
-        if (d[r] < dlo) continue;
-        const uint32_t k = (sh ? 0 : 2) + q;
-        xc[k] += 1;
-        xe[k] += d[r];
+        xc[0] += __popcll(b & ~bq);
+        xc[1] += __popcll(b & bq);
+        xc[2] += __popcll(bl & ~bq);
+        xc[3] += __popcll(bl & bq);
+        es[0] += sh && !q ? d[r] : 0u;
+        es[1] += sh && q ? d[r] : 0u;
       }
     }
     if (lane == 0) {
@@ -618,14 +625,13 @@ __global__ void __launch_bounds__(CBS, 4) k_bu_probe(ProbeArgs a) {
       nwords++;
     }
     // s[w][set * 4 + i]: i = 0, 1 entries (host, HBM), 2, 3 edges
+    const uint32_t e0 = wave_sum(es[0]), e1 = wave_sum(es[1]);
+    if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const uint32_t c = wave_sum(xc[k]);
-      const unsigned long long e = wave_sum(xe[k]);
-      if (lane == 0) {
-        s[wv][(k >> 1) * 4 + (k & 1)] = c;
-        s[wv][(k >> 1) * 4 + 2 + (k & 1)] = e;
-      }
+      for (int k = 0; k < 4; k++) s[wv][(k >> 1) * 4 + (k & 1)] = xc[k];
+      s[wv][2] = e0;
+      s[wv][3] = e1;
+      s[wv][6] = s[wv][7] = 0;
     }
     __syncthreads();
     if (threadIdx.x < 8) {

@@ -11,6 +11,15 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << lane_id()) - 1ull;
 }
 
+// The lanes where p holds, straight from the compare (HIP's __ballot takes
+// an int and may first turn the compare into a 0/1 value and back).
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Set bits of m in the lanes below this one (two mbcnt instructions).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Inclusive scan across the 64 lanes of a wave.
 template <class T>
 __device__ __forceinline__ T wave_incl_scan(T x) {
