@@ -397,14 +397,17 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
   // a top-down level's expansion, then its counters to the readback: slots
   // 0-4, this level's totals with the partitions its write pass touched and
   // the next frontier's sums (c_next, then reset)
-  auto expand_next = [&](LevelCtr *c, uint32_t level_value) {
+  // (one: the frontier is {src}, expanded from its list without a queue;
+  // the launch also writes c's totals and src's value)
+  auto expand_next = [&](LevelCtr *c, uint32_t level_value, bool one) {
     a.ctr = c;
     a.level_value = level_value;
     a.deg = g.d_deg.as<uint32_t>();
     a.rbit = g.store.d_rbit.as<uint32_t>();
     a.nctr = c_next;
     if (timed) HIPC(hipEventRecord(w.ev[0], s));
-    launch_expand(g, A_BFS_NEXT, a, s);
+    if (one) launch_expand_one(g, a, src, c, level_value - 1, s);
+    else launch_expand(g, A_BFS_NEXT, a, s);
     if (timed) HIPC(hipEventRecord(w.ev[1], s));
     LevelSide ts;
     ts.pub_src = d_ctr;
@@ -560,7 +563,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
     const bool fused = fused_next;
     // Level 0 from a source whose whole list would not pay for a probe (or,
     // all edges in HBM, would not pass the edge-ratio rule) runs top-down
-    // whatever its counts: its one-entry queue and the expansion, no
+    // whatever its counts: its list expanded in one launch, no queue, no
     // readback.
     const uint64_t d0 = L == 0 ? g.h_off[src + 1] - g.h_off[src] : 0;
     const bool quick =
@@ -568,11 +571,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
         (host_share ? (double)d0 * rec / link + td_hbm_cost(1, d0) <=
                           (double)g.n * PROBE_NS_VERTEX * 1e-9 + PROBE_FIXED
                     : (double)d0 <= (double)(g.m - std::min<uint64_t>(g.m, d0)) / g.do_alpha);
-    if (quick) {
-      // the frontier is {src}: its queue directly (the start cleared `dead`
-      // and c_next); the expansion's publish launch copies the counters out
-      launch_queue_one(g, w.qs, src, c, w.val.as<uint32_t>(), L + 1, s);
-    } else if (!fused) {
+    if (!quick && !fused) {
       LevelSide cs;
       cs.zero = dead;
       cs.nzero = words;
@@ -584,8 +583,10 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
                      0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &cs);
     }
     if (quick) {
+      // the frontier is {src}: its list expanded directly (the start cleared
+      // `dead` and c_next); the publish launch after it copies the counters
       a.next = next;
-      expand_next(c, L + 2);
+      expand_next(c, L + 2, true);
       visited_edges += d0;
       prev_nf = 1;
       pend = LevelRec();
@@ -688,7 +689,7 @@ static void bfs_direction_optimizing(Graph &g, uint32_t src, thv_stats *st) {
         launch_compact(g, w.qs, front, c, s, probed || fused ? 0 : CMP_WRITE_ONLY, nullptr, 0,
                        1, 0xFFFFFFFFu, 0, nullptr, 0, nullptr, nullptr, &zs);
       }
-      expand_next(c, L + 2);
+      expand_next(c, L + 2, false);
     } else {
       if (!probed) {
         summary();

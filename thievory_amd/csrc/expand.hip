@@ -46,6 +46,53 @@ __device__ __forceinline__ uint32_t hub_slot(uint32_t w) {
   return (w * 2654435761u) >> (32 - __builtin_ctz(PR_HUB_SLOTS));
 }
 
+// BFS relaxation of EPT edges whose destinations are loaded (an absent edge
+// is 0xFFFFFFFF), each phase issuing the memory operations of all EPT edges
+// before any result is used: the visited words, the returning atomics of
+// the unvisited ones, the next-frontier bits; with NEXT, the winners' degree
+// and residency words, their levels and the next frontier's sums.
+template <int EPT, bool NEXT>
+__device__ __forceinline__ void bfs_relax(const ExpandArgs &a, const uint32_t (&dst)[EPT],
+                                          unsigned long long (&nfn)[2],
+                                          unsigned long long (&nfe)[2]) {
+  uint32_t vw[EPT], old[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; j++) vw[j] = a.vis[dst[j] != 0xFFFFFFFFu ? dst[j] >> 5 : 0];
+#pragma unroll
+  for (int j = 0; j < EPT; j++) {
+    const uint32_t w = dst[j], bit = 1u << (w & 31);
+    old[j] = 0xFFFFFFFFu;
+    if (w != 0xFFFFFFFFu && !(vw[j] & bit)) old[j] = atomicOr(&a.vis[w >> 5], bit);
+  }
+  // the winner marks the next frontier; without NEXT the next level's
+  // compaction writes its level, with NEXT the winner writes it and adds to
+  // the sums (its degree and residency words loaded together)
+#pragma unroll
+  for (int j = 0; j < EPT; j++) {
+    const uint32_t w = dst[j], bit = 1u << (w & 31);
+    if (!(old[j] & bit)) atomicOr(&a.next[w >> 5], bit);
+  }
+  if (NEXT) {
+    uint32_t dw[EPT], rw[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      const uint32_t w = dst[j];
+      const bool won = !(old[j] & (1u << (w & 31)));
+      dw[j] = won ? a.deg[w] : 0u;
+      rw[j] = won ? a.rbit[w >> 5] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < EPT; j++) {
+      const uint32_t w = dst[j];
+      if (old[j] & (1u << (w & 31))) continue;
+      a.val[w] = a.level_value;
+      const uint32_t q = (rw[j] >> (w & 31)) & 1u;
+      nfn[q] += 1;
+      nfe[q] += dw[j];
+    }
+  }
+}
+
 template <int ALGO, int REC, int EPT>
 __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
   constexpr uint32_t C = EBS * EPT;
@@ -148,42 +195,7 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
     // absent edge reads word 0 and is ignored), then the returning atomics
     // of the edges that need one, then the next-frontier bits.
     if (ALGO == A_BFS || ALGO == A_BFS_NEXT) {
-      uint32_t vw[EPT], old[EPT];
-#pragma unroll
-      for (int j = 0; j < EPT; j++) vw[j] = a.vis[dst[j] != 0xFFFFFFFFu ? dst[j] >> 5 : 0];
-#pragma unroll
-      for (int j = 0; j < EPT; j++) {
-        const uint32_t w = dst[j], bit = 1u << (w & 31);
-        old[j] = 0xFFFFFFFFu;
-        if (w != 0xFFFFFFFFu && !(vw[j] & bit)) old[j] = atomicOr(&a.vis[w >> 5], bit);
-      }
-      // the winner marks the next frontier; A_BFS leaves the level to the
-      // next level's compaction, A_BFS_NEXT writes it and sums the winners
-      // (their degree and residency words loaded together)
-#pragma unroll
-      for (int j = 0; j < EPT; j++) {
-        const uint32_t w = dst[j], bit = 1u << (w & 31);
-        if (!(old[j] & bit)) atomicOr(&a.next[w >> 5], bit);
-      }
-      if (ALGO == A_BFS_NEXT) {
-        uint32_t dw[EPT], rw[EPT];
-#pragma unroll
-        for (int j = 0; j < EPT; j++) {
-          const uint32_t w = dst[j];
-          const bool won = !(old[j] & (1u << (w & 31)));
-          dw[j] = won ? a.deg[w] : 0u;
-          rw[j] = won ? a.rbit[w >> 5] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < EPT; j++) {
-          const uint32_t w = dst[j];
-          if (old[j] & (1u << (w & 31))) continue;
-          a.val[w] = a.level_value;
-          const uint32_t q = (rw[j] >> (w & 31)) & 1u;
-          nfn[q] += 1;
-          nfe[q] += dw[j];
-        }
-      }
+      bfs_relax<EPT, ALGO == A_BFS_NEXT>(a, dst, nfn, nfe);
     } else if (ALGO == A_CC || ALGO == A_SSSP) {
       uint32_t cur[EPT], old[EPT], nv[EPT];
 #pragma unroll
@@ -278,6 +290,77 @@ __global__ void __launch_bounds__(EBS) k_expand(ExpandArgs a) {
       if (y1) atomicAdd(&a.nctr->qw[1], y1);
     }
   }
+}
+
+// The expansion of one vertex's list (a BFS level whose frontier is {v},
+// with the next-frontier sums): no queue, chunk table or staging -- block
+// b takes records [b * 1024, ...) of the list, four per thread, in
+// grid-stride steps (loaded clamped to the list, so all four are in flight
+// together).  Block 0 also writes ctr's totals for the one-entry frontier
+// (its residency from rbit, its partition's from vpart and pres) and v's
+// value, as the queue kernels would.
+constexpr int EPT_ONE = 4;
+template <int REC>
+__global__ void __launch_bounds__(EBS) k_expand_one(ExpandArgs a, uint32_t v, uint32_t d,
+                                                    const uint32_t *vpart, const uint8_t *pres,
+                                                    LevelCtr *ctr, uint32_t v_value) {
+  unsigned long long nfn[2] = {0, 0}, nfe[2] = {0, 0};
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
+    const int tq = pres[vpart[v]] ? Q_DEV : Q_HOST;
+    LevelCtr t = *ctr;  // keeps cand
+    t.qn[q] = 1;
+    t.qn[q ^ 1] = 0;
+    t.qw[q] = d;
+    t.qw[q ^ 1] = 0;
+    t.touched[tq] = 1;
+    t.touched[tq ^ 1] = 0;
+    t.rd[0] = t.rd[1] = 0;
+    *ctr = t;
+    a.val[v] = v_value;
+  }
+  const unsigned long long eb = a.eptr[v];
+  for (uint64_t i0 = (uint64_t)blockIdx.x * EBS * EPT_ONE; i0 < d;
+       i0 += (uint64_t)gridDim.x * EBS * EPT_ONE) {
+    uint32_t dst[EPT_ONE];
+#pragma unroll
+    for (int j = 0; j < EPT_ONE; j++) {
+      const uint64_t i = i0 + threadIdx.x + (uint64_t)j * EBS;
+      const uint32_t x = *reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(
+          eb + (i < d ? i : d - 1) * REC);
+      dst[j] = i < d ? x : 0xFFFFFFFFu;
+    }
+    bfs_relax<EPT_ONE, true>(a, dst, nfn, nfe);
+  }
+  __shared__ unsigned long long s_n[EBS / 64];
+  const unsigned long long x0 = block_sum<EBS>(nfn[0], s_n), x1 = block_sum<EBS>(nfn[1], s_n);
+  const unsigned long long y0 = block_sum<EBS>(nfe[0], s_n), y1 = block_sum<EBS>(nfe[1], s_n);
+  if (threadIdx.x == 0) {
+    if (x0) atomicAdd(&a.nctr->qn[0], x0);
+    if (x1) atomicAdd(&a.nctr->qn[1], x1);
+    if (y0) atomicAdd(&a.nctr->qw[0], y0);
+    if (y1) atomicAdd(&a.nctr->qw[1], y1);
+  }
+}
+
+void launch_expand_one(Graph &g, const ExpandArgs &args, uint32_t v, LevelCtr *ctr,
+                       uint32_t v_value, hipStream_t s) {
+  if (v >= g.n) fail(THV_EINVAL, "vertex out of range");
+  const uint64_t d = g.h_off[v + 1] - g.h_off[v];
+  if (d == 0 || d > 0xFFFFFFFFull) fail(THV_ESTATE, "one-list expansion of 1..2^32-1 records");
+  if (!args.nctr || !args.rbit || !args.deg || !args.val || !args.vis || !args.next)
+    fail(THV_ESTATE, "one-list expansion needs nctr, rbit, deg, val and the bitmaps");
+  const uint32_t *vpart = g.store.d_vpart.as<uint32_t>();
+  const uint8_t *pres = g.store.d_pres.as<uint8_t>();
+  const uint64_t blocks = div_up(d, (uint64_t)EBS * EPT_ONE);
+  const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)g.shape.blocks_per_cu * g.cus);
+  if (g.rec == 8)
+    hipLaunchKernelGGL(k_expand_one<8>, dim3(grid), dim3(EBS), 0, s, args, v, (uint32_t)d, vpart,
+                       pres, ctr, v_value);
+  else
+    hipLaunchKernelGGL(k_expand_one<4>, dim3(grid), dim3(EBS), 0, s, args, v, (uint32_t)d, vpart,
+                       pres, ctr, v_value);
+  HIPC(hipGetLastError());
 }
 
 template <int ALGO, int REC>

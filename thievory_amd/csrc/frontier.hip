@@ -970,29 +970,6 @@ __global__ void __launch_bounds__(CBS) k_touched(CompactArgs a) {
   }
 }
 
-// launch_queue_one: thread 0 writes the entry and the totals, the block the
-// chunk starts (every chunk of the list begins in entry 0).
-__global__ void __launch_bounds__(CBS) k_queue_one(CompactArgs a, uint32_t v) {
-  const uint32_t d = a.deg[v];
-  const int q = (a.rbit[v >> 5] >> (v & 31)) & 1u ? Q_DEV : Q_HOST;
-  if (threadIdx.x == 0) {
-    a.qv[q][0] = v;
-    a.qe[q][0] = 0;
-    if (a.val) a.val[v] = a.val_value;
-    LevelCtr t = *a.ctr;  // keeps cand
-    t.qn[q] = 1;
-    t.qn[q ^ 1] = 0;
-    t.qw[q] = d;
-    t.qw[q ^ 1] = 0;
-    const int tq = a.pres[a.vpart[v]] ? Q_DEV : Q_HOST;
-    t.touched[tq] = 1;
-    t.touched[tq ^ 1] = 0;
-    t.rd[0] = t.rd[1] = 0;
-    *a.ctr = t;
-  }
-  for (uint64_t c = threadIdx.x; c * a.C < d; c += CBS) a.qc[q][c] = 0;
-}
-
 // A sparse frontier's queue (launch_queue_sparse).  k_list_front: one pass
 // over the bitmap, four words per thread, appends the frontier vertices to
 // `list` (one atomic per wave that holds any; the caller bounds the
@@ -1114,29 +1091,6 @@ void launch_fsum(Graph &g, const uint32_t *front, uint32_t *fsum, hipStream_t s)
   const uint32_t grid = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>(div_up(nbits, CBS), 8ull * g.cus));
   hipLaunchKernelGGL(k_fsum, dim3(grid), dim3(CBS), 0, s, front, bitmap_words(g.n), fsum, nbits);
-  HIPC(hipGetLastError());
-}
-
-void launch_queue_one(Graph &g, QueueSet &w, uint32_t v, LevelCtr *ctr, uint32_t *val,
-                      uint32_t val_value, hipStream_t s) {
-  if (v >= g.n || g.h_off[v + 1] == g.h_off[v])
-    fail(THV_ESTATE, "a one-vertex queue needs a vertex with edges");
-  CompactArgs a;
-  memset(&a, 0, sizeof a);
-  a.rbit = g.store.d_rbit.as<uint32_t>();
-  a.deg = g.d_deg.as<uint32_t>();
-  a.vpart = g.store.d_vpart.as<uint32_t>();
-  a.pres = g.store.d_pres.as<uint8_t>();
-  for (int q = 0; q < 2; q++) {
-    a.qv[q] = w.qv[q].as<uint32_t>();
-    a.qe[q] = w.qe[q].as<unsigned long long>();
-    a.qc[q] = w.qc[q].as<uint32_t>();
-  }
-  a.ctr = ctr;
-  a.C = w.C;
-  a.val = val;
-  a.val_value = val_value;
-  hipLaunchKernelGGL(k_queue_one, dim3(1), dim3(CBS), 0, s, a, v);
   HIPC(hipGetLastError());
 }
 

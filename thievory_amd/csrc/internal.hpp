@@ -321,12 +321,6 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 // *minout to the smallest fval of a selected vertex.  side (count pass
 // only; ignored by WRITE_ONLY): the count kernel clears side->zero, the
 // scan publishes after writing the totals and resets side->zctr.
-// The queue of a one-vertex frontier {v} (degree >= 1) without a pass over
-// the bitmap: what launch_compact would write for it -- the entry, its edge
-// offset and chunk starts, and ctr's totals with v's partition as the one
-// touched (ctr's cand kept) -- and val[v] = val_value.  One small launch.
-void launch_queue_one(Graph &g, QueueSet &qs, uint32_t v, LevelCtr *ctr, uint32_t *val,
-                      uint32_t val_value, hipStream_t s);
 // The queue of a frontier of at most SPARSE_MAX vertices (the caller knows
 // its size) without tile tables: one pass over the bitmap lists its
 // vertices (into qs.slist, counted in ctr->pad[0], which must be 0 and is
@@ -378,6 +372,11 @@ struct BottomUpArgs {
   const uint32_t *fsum = nullptr;  // frontier summary (launch_fsum), nullable
 };
 void launch_bottom_up(Graph &g, const BottomUpArgs &a, hipStream_t s);
+// The BFS expansion of the frontier {v} (1 <= degree < 2^32) with the
+// next-frontier sums (args as for A_BFS_NEXT): one launch, no queue; it
+// also writes ctr's totals for {v} and val[v] = v_value.
+void launch_expand_one(Graph &g, const ExpandArgs &args, uint32_t v, LevelCtr *ctr,
+                       uint32_t v_value, hipStream_t s);
 // One wave copies side's counters to the host-mapped readback (no zero
 // words), after the launches before it on the stream.
 void launch_publish(const LevelSide &side, hipStream_t s);
