@@ -608,6 +608,36 @@ def test_hub_spanning_many_chunks():
         assert np.array_equal(g.cc()[0], O.cc(og))
 
 
+def test_symmetric_star_path_small_frontiers():
+    """Direction-optimising BFS on a symmetric star (hub 0, 20,000 leaves)
+    with a 2,000-vertex path hanging off one leaf: from a leaf, level 0 is a
+    one-list expansion, level 1's frontier is the hub alone (a listed queue
+    whose one list spans many expansion chunks), then the leaves, then one
+    path vertex per level (listed queues); from the hub and from the path's
+    end as well, all edges in HBM and all on the host."""
+    _say("case: symmetric star + path")
+    n = 22000
+    adj = [[] for _ in range(n)]
+    for v in range(1, 20000):
+        adj[0].append(v)
+        adj[v].append(0)
+    for v in range(19999, n - 1):
+        adj[v].append(v + 1)
+        adj[v + 1].append(v)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(sorted(a)) for a in adj])
+    dst = np.array([x for a in adj for x in sorted(a)], np.uint32)
+    og = O.CSR(off, dst)
+    g = T.Graph.from_arrays(off, dst)
+    g.set_option("symmetric", 1)
+    for frac in (1.0, 0.0):
+        g.store_config(int(frac * g.m * 4), 512)
+        for s in (5, 0, n - 1, 19999):
+            lv, st = g.bfs(s)
+            assert np.array_equal(lv, O.bfs(og, s)), (frac, s)
+            assert st.edges_traversed == int(og.degrees()[lv != n + 1].sum()), (frac, s)
+
+
 def test_directed_cc_is_min_ancestor():
     _say("case: directed cc")
     og = O.rmat(12, 8, seed=9, symmetric=False)
