@@ -723,8 +723,8 @@ __global__ void __launch_bounds__(CBS) k_bu_probe_list(ProbeArgs a) {
         const uint32_t oJ = (uint32_t)__shfl((int)oj, jj, 64);
         v[u] = t < T ? a.cin[(uint64_t)(r0 + jj) * SUB + (t - oJ)] : VNONE;
       }
-      // a lane past the candidates reads the words of vertex 0 (the hint
-      // and degree arrays are padded) and ignores them
+      // a lane past the candidates reads the words of vertex 0 (always a
+      // valid vertex) and ignores them
       uint32_t vw[LU], rw[LU], hr[LU], dr[LU];
 #pragma unroll
       for (uint32_t u = 0; u < LU; u++) {

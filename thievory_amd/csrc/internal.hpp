@@ -321,6 +321,13 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 // *minout to the smallest fval of a selected vertex.  side (count pass
 // only; ignored by WRITE_ONLY): the count kernel clears side->zero, the
 // scan publishes after writing the totals and resets side->zctr.
+void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
+                    hipStream_t s, int mode = 0, uint32_t *val = nullptr,
+                    uint32_t val_value = 0, uint32_t dmin = 1,
+                    uint32_t dmax = 0xFFFFFFFFu, int tset = 0,
+                    const uint32_t *fval = nullptr, uint32_t fthr = 0,
+                    uint32_t *clear = nullptr, uint32_t *minout = nullptr,
+                    const LevelSide *side = nullptr);
 // The queue of a frontier of at most SPARSE_MAX vertices (the caller knows
 // its size) without tile tables: one pass over the bitmap lists its
 // vertices (into qs.slist, counted in ctr->pad[0], which must be 0 and is
@@ -329,13 +336,6 @@ enum { CMP_INVERT = 1, CMP_NO_CHUNKS = 2, CMP_COUNT_ONLY = 4, CMP_WRITE_ONLY = 8
 constexpr uint32_t SPARSE_MAX = 4096;
 void launch_queue_sparse(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
                          hipStream_t s, const LevelSide *side);
-void launch_compact(Graph &g, QueueSet &qs, const uint32_t *front, LevelCtr *ctr,
-                    hipStream_t s, int mode = 0, uint32_t *val = nullptr,
-                    uint32_t val_value = 0, uint32_t dmin = 1,
-                    uint32_t dmax = 0xFFFFFFFFu, int tset = 0,
-                    const uint32_t *fval = nullptr, uint32_t fthr = 0,
-                    uint32_t *clear = nullptr, uint32_t *minout = nullptr,
-                    const LevelSide *side = nullptr);
 // Bottom-up probe, one sweep over the vertices: every unvisited vertex
 // whose parent hint (hint may be null) is in front joins vis and next, and
 // so does every unvisited vertex of degree 1..D with a frontier vertex in

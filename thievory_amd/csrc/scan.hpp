@@ -11,8 +11,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << lane_id()) - 1ull;
 }
 
-// The lanes where p holds, straight from the compare (HIP's __ballot takes
-// an int and may first turn the compare into a 0/1 value and back).
+// The lanes where p holds (a 64-bit ballot of a bool).
 __device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Set bits of m in the lanes below this one (two mbcnt instructions).
