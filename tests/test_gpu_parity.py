@@ -638,6 +638,43 @@ def test_symmetric_star_path_small_frontiers():
             assert st.edges_traversed == int(og.degrees()[lv != n + 1].sum()), (frac, s)
 
 
+@pytest.mark.parametrize("k", [4096, 4097])
+def test_bfs_listed_queue_limit(k):
+    """Top-down levels of exactly 4,096 frontier vertices (listed from the
+    bitmap in one pass) and of 4,097 (the tile-table compaction): a hub 0
+    with k spokes, each spoke with one more neighbour.  With every list and
+    the parent hints in HBM (the default store) the edge-ratio rule keeps
+    the spokes' level top-down (do_alpha tiny; only a level that finds
+    every edge already visited may run bottom-up); with every list on the
+    host the link-cost rule chooses, and the levels must match either way."""
+    _say("case: listed queue limit", k)
+    n = 2 * k + 1
+    adj = [[] for _ in range(n)]
+    for i in range(1, k + 1):
+        adj[0].append(i)
+        adj[i] += [0, k + i]
+        adj[k + i].append(i)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(a) for a in adj])
+    dst = np.array([x for a in adj for x in sorted(a)], np.uint32)
+    og = O.CSR(off, dst)
+    g = T.Graph.from_arrays(off, dst)
+    g.set_option("symmetric", 1)
+    g.set_option("do_alpha", 1e-9)
+    try:
+        for budget in (None, 0):
+            g.store_config(budget, 256)
+            assert (g.store_info()["device_edges"] == g.m) == (budget is None)
+            for s in (0, 1, n - 1):
+                lv, st = g.bfs(s)
+                assert np.array_equal(lv, O.bfs(og, s)), (budget, s)
+                if budget is None:
+                    assert st.bottom_up_levels <= 1, s
+                assert st.edges_traversed == int(og.degrees()[lv != n + 1].sum()), (budget, s)
+    finally:
+        g.set_option("do_alpha", 14.0)
+
+
 def test_directed_cc_is_min_ancestor():
     _say("case: directed cc")
     og = O.rmat(12, 8, seed=9, symmetric=False)
