@@ -7,9 +7,9 @@
 // top-down's (the reference only has the top-down rule, bfs_kernels.cu:
 // 29-44); what changes is how many edge records cross the host link.
 //
-// One group of 16 lanes per unvisited vertex with a long list (the probe
+// One group of G = 8 lanes per unvisited vertex with a long list (the probe
 // wrote them per wave region, in vertex order, and k_lbase the regions'
-// bases): the group reads 16 consecutive records (64 bytes of 4-byte
+// bases): the group reads 8 consecutive records (32 bytes of 4-byte
 // records) per step and tests the frontier bitmap; a ballot ends the scan
 // at the first hit.  Every wave takes an equal run of the lists in region
 // order, wherever they cluster (on R-MAT graphs the long lists crowd into
@@ -26,7 +26,10 @@ namespace thv {
 namespace {
 
 constexpr int BBS = 256;
-constexpr int G = 16;  // lanes per vertex
+// lanes per vertex: 8 measured best at R-MAT 26 (64 random roots 2.2% less
+// time than with 16, vertex 1 the same; 4 and 32 slower)
+constexpr int G = 8;
+constexpr unsigned long long GM = (1ull << G) - 1ull;  // a group's ballot bits
 
 template <int REC, bool SUM>
 __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
@@ -56,11 +59,11 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
     for (uint64_t i0 = s; i0 < e; i0 += 64 / G) {
       const uint64_t i = i0 + gw;
       if (i >= e) break;
-      // move on to the region holding list i, 16 bases per step
+      // move on to the region holding list i, G bases per step
       for (;;) {
         const uint64_t r = (uint64_t)rg + 1 + gl;
         const bool le = r <= a.nregions && a.loff[r] <= i;
-        const uint32_t c = (uint32_t)__popcll((__ballot(le) >> (gw * G)) & 0xFFFFull);
+        const uint32_t c = (uint32_t)__popcll((__ballot(le) >> (gw * G)) & GM);
         rg += c;
         if (c < G) break;
       }
@@ -80,7 +83,7 @@ __global__ void __launch_bounds__(BBS) k_bottom_up(BottomUpArgs a) {
           hit = maybe && ((a.front[w >> 5] >> (w & 31)) & 1u);
         }
         const unsigned long long b = __ballot(hit);
-        if ((b >> (gw * G)) & 0xFFFFull) {
+        if ((b >> (gw * G)) & GM) {
           found = true;
           break;
         }
